@@ -518,13 +518,13 @@ static void build_stencils(orc_system *S, int level) {
   lvl_t *L = &S->lv[level];
   const lvl_t *L0 = &S->lv[0];
   int depth = level;
-  int ncell = 1 << depth;
-  int64_t npos = (int64_t)ncell * ncell * ncell;
+  int ncell_axis = 1 << depth;
+  int64_t npos = (int64_t)ncell_axis * ncell_axis * ncell_axis;
   double *Mmu = (double *)malloc((size_t)npos * 576 * sizeof(double));
   double *Mla = (double *)malloc((size_t)npos * 576 * sizeof(double));
 #pragma omp parallel for schedule(static)
   for (int64_t q = 0; q < npos; ++q) {
-    int p[3] = {(int)(q / (ncell * ncell)), (int)((q / ncell) % ncell), (int)(q % ncell)};
+    int p[3] = {(int)(q / (ncell_axis * ncell_axis)), (int)((q / ncell_axis) % ncell_axis), (int)(q % ncell_axis)};
     position_matrix(S->kmu, depth, p, Mmu + q * 576);
     position_matrix(S->kla, depth, p, Mla + q * 576);
   }
@@ -532,32 +532,43 @@ static void build_stencils(orc_system *S, int level) {
   L->spoke = (uint32_t *)calloc((size_t)L->nn, sizeof(uint32_t));
   L->nb_stencil = 1;
   double dx = S->p.dx;
+  /* each coarse cell's 24x24 matrix once (cell at its allocated min-corner node) */
+  int32_t *crow = (int32_t *)malloc((size_t)L->nn * sizeof(int32_t));
+  int64_t ncell = 0;
+  for (int64_t n = 0; n < L->nn; ++n) {
+    int64_t bl = box_lin(L, L->xyz[3 * n], L->xyz[3 * n + 1], L->xyz[3 * n + 2]);
+    crow[n] = (L->bflags[bl] & ORC_ACTIVE_CELL) ? (int32_t)ncell++ : -1;
+  }
+  double *CA = (double *)calloc((size_t)(ncell > 0 ? ncell : 1) * 576, sizeof(double));
+  uint8_t *cany = (uint8_t *)calloc((size_t)(ncell > 0 ? ncell : 1), 1);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t n = 0; n < L->nn; ++n) {
+    if (crow[n] < 0) continue;
+    double *A = CA + (int64_t)crow[n] * 576;
+    int f0[3];
+    for (int d = 0; d < 3; ++d) f0[d] = ncell_axis * L->xyz[3 * n + d] - (ncell_axis - 1) * BPAD[d];
+    for (int64_t q = 0; q < npos; ++q) {
+      int p[3] = {(int)(q / (ncell_axis * ncell_axis)), (int)((q / ncell_axis) % ncell_axis), (int)(q % ncell_axis)};
+      int32_t c = node_at(L0, f0[0] + p[0], f0[1] + p[1], f0[2] + p[2]);
+      if (c < 0) continue;
+      double mu = S->mu[c], la = S->la[c];
+      if (mu == 0.0 && la == 0.0) continue;
+      cany[crow[n]] = 1;
+      const double *mm = Mmu + q * 576, *ml = Mla + q * 576;
+      for (int e = 0; e < 576; ++e) A[e] += mu * mm[e] + la * ml[e];
+    }
+  }
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t n = 0; n < L->nn; ++n) {
     if (!(L->flags[n] & ORC_ACTIVE_NODE)) continue;
     double *st = L->st + n * 243;
     uint32_t spk = 0;
-    double A[576];
     for (int pc = 0; pc < 8; ++pc) {
       int px = (pc >> 2) & 1, py = (pc >> 1) & 1, pz = pc & 1;
-      int C[3] = {L->xyz[3 * n] + px - 1, L->xyz[3 * n + 1] + py - 1, L->xyz[3 * n + 2] + pz - 1};
-      /* fine cells of coarse cell C: min corner 2^d C - (2^d - 1) B + p */
-      int f0[3];
-      for (int d = 0; d < 3; ++d) f0[d] = ncell * C[d] - (ncell - 1) * BPAD[d];
-      memset(A, 0, sizeof(A));
-      int any = 0;
-      for (int64_t q = 0; q < npos; ++q) {
-        int p[3] = {(int)(q / (ncell * ncell)), (int)((q / ncell) % ncell), (int)(q % ncell)};
-        int32_t c = node_at(L0, f0[0] + p[0], f0[1] + p[1], f0[2] + p[2]);
-        if (c < 0) continue;
-        double mu = S->mu[c], la = S->la[c];
-        if (mu == 0.0 && la == 0.0) continue;
-        any = 1;
-        const double *mm = Mmu + q * 576, *ml = Mla + q * 576;
-        for (int e = 0; e < 576; ++e) A[e] += mu * mm[e] + la * ml[e];
-      }
-      if (!any) continue;
-      int I = ((1 - px) << 2) | ((1 - py) << 1) | (1 - pz); /* our node within C */
+      int32_t cn = node_at(L, L->xyz[3 * n] + px - 1, L->xyz[3 * n + 1] + py - 1, L->xyz[3 * n + 2] + pz - 1);
+      if (cn < 0 || crow[cn] < 0 || !cany[crow[cn]]) continue;
+      const double *A = CA + (int64_t)crow[cn] * 576;
+      int I = ((1 - px) << 2) | ((1 - py) << 1) | (1 - pz); /* our node within the cell */
       for (int J = 0; J < 8; ++J) {
         int jx = (J >> 2) & 1, jy = (J >> 1) & 1, jz = J & 1;
         int s = (px + jx) * 9 + (py + jy) * 3 + (pz + jz);
@@ -574,6 +585,9 @@ static void build_stencils(orc_system *S, int level) {
     for (int e = 0; e < 243; ++e) st[e] *= dx;
     L->spoke[n] = spk;
   }
+  free(CA);
+  free(cany);
+  free(crow);
   /* Delete_Dirichlet_Equations: MG/Delete_Dirichlet_Equations.h:24-51 */
 #pragma omp parallel for schedule(static)
   for (int64_t n = 0; n < L->nn; ++n) {

@@ -1,0 +1,98 @@
+// kernels.hpp -- launch wrappers of the gfx950 kernels (kernels.hip).
+//
+// Level data layout (device): blocks of 4x4x4 nodes, local index
+// l = i*16 + j*4 + k; a vector is stored per block and component:
+// v[(blk*3 + c)*64 + l]; explicit-level stencils ST[(blk*243 + e)*64 + l]
+// with e = ((s*3 + v)*3 + w), s = (di+1)*9 + (dj+1)*3 + (dk+1).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spg {
+
+struct LevelDev {
+  int nb = 0;
+  const int* nbr = nullptr;        // [nb*27]
+  const uint8_t* flags = nullptr;  // [nb*64]
+  const int* rsrc = nullptr;       // [nb*27] restriction sources (level >= 1)
+  const int* psrc = nullptr;       // [nb*8]  prolongation sources (level < L-1)
+  const int* base = nullptr;       // [nb*3]  block origin (node coords)
+  const int* bnd[8] = {};          // boundary nodes per colour
+  int nbnd[8] = {};
+};
+
+// device scalars of one PCG iteration
+struct CgScalars {
+  double rho, rho_old, sq, alpha, beta, nrm, nrm2;
+  int singular;
+  int pad;
+};
+
+void upload_k0(const double* K0, hipStream_t st);
+
+// level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
+template <class T>
+void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode);
+// level-0 GS: one colour over all blocks (boundary=false) or over the boundary list
+template <class T>
+void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour, bool boundary);
+// explicit levels
+template <class T>
+void launch_st_residual(hipStream_t st, const LevelDev& L, const T* ST, const T* u, const T* b, T* r);
+template <class T>
+void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int colour, bool boundary);
+// transfers
+template <class T>
+void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse);
+template <class T>
+void launch_prolongate(hipStream_t st, const LevelDev& F, const T* coarse, T* fine);
+// vectors
+template <class T>
+void launch_zero(hipStream_t st, T* v, int64_t n);
+template <class T>
+void launch_copy(hipStream_t st, const T* a, T* b, int64_t n);
+void launch_cast_d2f(hipStream_t st, const double* a, float* b, int64_t n);
+void launch_cast_f2d(hipStream_t st, const float* a, double* b, int64_t n);
+template <class T>
+void launch_axpy_d(hipStream_t st, double* u, const T* d, int64_t n);  // u += d (FP64 accumulate)
+// reductions (two-stage, deterministic); partial buffer needs >= reduce_partials(n) doubles
+int64_t reduce_partials(int64_t n);
+template <class T>
+void launch_dot(hipStream_t st, const T* a, const T* b, int64_t n, double* partial, double* out);
+template <class T>
+void launch_linf(hipStream_t st, const T* a, int64_t n, double* partial, double* out);
+void launch_sumsq(hipStream_t st, const double* a, int64_t n, double* partial, double* out);
+// PCG steps (scalars on device)
+template <class T>
+void launch_cg_direction(hipStream_t st, T* s, const T* z, const CgScalars* sc, int64_t n);  // s = beta s + z
+template <class T>
+void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, const CgScalars* sc, int64_t n);
+void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage);  // 0: after rho, 1: after sq, 2: end
+// projections
+template <class T>
+void launch_project(hipStream_t st, const LevelDev& L, T* v, bool nullspace);
+// Galerkin assembly (FP64)
+void launch_galerkin_gather(hipStream_t st, int64_t ncell, int depth, const int* cell_fine_blocks,
+                            const int* cell_gpu_index, const double* E0, double* G);
+void launch_galerkin_assemble(hipStream_t st, const LevelDev& L, const int* cell_row, const double* Cm, double dx,
+                              double* ST);
+void launch_stencil_to_float(hipStream_t st, const double* a, float* b, int64_t n);
+// bottom
+void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
+                           double* A);
+void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n);
+template <class T>
+void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, double* x);
+template <class T>
+void launch_bottom_scatter(hipStream_t st, const LevelDev& L, const int* var_of, const double* y, const T* b, T* u);
+// sensitivity: per level-0 slot (cell): dc = -p rho^(p-1)(1-Emin) a_e^T K0 u_e ; obj partials
+void launch_sensitivity(hipStream_t st, const LevelDev& L, const double* rho, const double* u, const double* a,
+                        double penalty, double emin, double* dc, double* objcell);
+// gather / scatter between input-block order and level-0 slots
+void launch_scatter_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* in3, double* v);
+void launch_gather_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out3);
+void launch_gather_input_scalar(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out);
+void launch_set_entries(hipStream_t st, double* v, const int* idx, const double* val, int64_t n);
+
+}  // namespace spg

@@ -1,0 +1,455 @@
+// layout.cpp -- see layout.hpp.
+#include "layout.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <unordered_set>
+
+namespace spg {
+
+double stiffness_mapping(double penalty, double min_stiffness, double min_density, double d) {
+  // topo_opt/fem_interface.h:343-353
+  if (penalty != 0.0) return (1.0 - min_stiffness) * std::pow(std::max(min_density, d), penalty) + min_stiffness;
+  return d;
+}
+
+// ------------------------------------------------------------- element tables
+// Q1 hex on the unit cube, exact integrals as tensor products of the 1-D
+// integrals of the linear shape functions phi_0 = 1-t, phi_1 = t:
+//   mass m_ab = int phi_a phi_b, stiffness k_ab = int phi'_a phi'_b,
+//   mixed g_ab = int phi'_a phi_b.
+// K_mu[a][b][v][w] = delta_vw sum_d int dN_a/dx_d dN_b/dx_d + int dN_a/dx_w dN_b/dx_v
+// K_la[a][b][v][w] = int dN_a/dx_v dN_b/dx_w
+// (values match solver/CPU_Kernels/Stiffness_Matrices.h:8-688).
+static double grad_product(int a, int b, int p, int q) {
+  static const double m[2][2] = {{1.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 1.0 / 3.0}};
+  static const double k[2][2] = {{1.0, -1.0}, {-1.0, 1.0}};
+  static const double g[2][2] = {{-0.5, -0.5}, {0.5, 0.5}};  // g[a][b] = int phi'_a phi_b
+  double r = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    int ad = (a >> (2 - d)) & 1, bd = (b >> (2 - d)) & 1;
+    if (d == p && d == q) r *= k[ad][bd];
+    else if (d == p) r *= g[ad][bd];
+    else if (d == q) r *= g[bd][ad];
+    else r *= m[ad][bd];
+  }
+  return r;
+}
+
+void element_tables(double* kmu, double* kla) {
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b)
+      for (int v = 0; v < 3; ++v)
+        for (int w = 0; w < 3; ++w) {
+          double lap = 0.0;
+          if (v == w) lap = grad_product(a, b, 0, 0) + grad_product(a, b, 1, 1) + grad_product(a, b, 2, 2);
+          int e = ((a * 8 + b) * 3 + v) * 3 + w;
+          kmu[e] = lap + grad_product(a, b, w, v);
+          kla[e] = grad_product(a, b, v, w);
+        }
+}
+
+std::vector<double> galerkin_position_matrices(int depth, const double* K0) {
+  const int n = 1 << depth;
+  const int64_t npos = (int64_t)n * n * n;
+  std::vector<double> out((size_t)npos * 576);
+  for (int64_t q = 0; q < npos; ++q) {
+    int p[3] = {(int)(q / (n * n)), (int)((q / n) % n), (int)(q % n)};
+    double W[8][8];  // weight of coarse node A at fine node a
+    for (int a = 0; a < 8; ++a)
+      for (int A = 0; A < 8; ++A) {
+        double w = 1.0;
+        for (int d = 0; d < 3; ++d) {
+          double t = (double)(p[d] + ((a >> (2 - d)) & 1)) / (double)n;
+          w *= ((A >> (2 - d)) & 1) ? t : 1.0 - t;
+        }
+        W[a][A] = w;
+      }
+    // T[a][B][v][w] = sum_b K[a][b][v][w] W[b][B]
+    double T[8][8][9];
+    for (int a = 0; a < 8; ++a)
+      for (int B = 0; B < 8; ++B)
+        for (int e = 0; e < 9; ++e) {
+          double s = 0.0;
+          for (int b = 0; b < 8; ++b) s += K0[(a * 8 + b) * 9 + e] * W[b][B];
+          T[a][B][e] = s;
+        }
+    double* M = out.data() + q * 576;
+    for (int A = 0; A < 8; ++A)
+      for (int B = 0; B < 8; ++B)
+        for (int e = 0; e < 9; ++e) {
+          double s = 0.0;
+          for (int a = 0; a < 8; ++a) s += W[a][A] * T[a][B][e];
+          M[(A * 8 + B) * 9 + e] = s;
+        }
+  }
+  return out;
+}
+
+// ------------------------------------------------------------- host levels
+int32_t HostLevel::get(int bx, int by, int bz) {
+  uint64_t k = bkey(bx, by, bz);
+  auto it = map.find(k);
+  if (it != map.end()) return it->second;
+  int32_t id = (int32_t)alloc.size();
+  map.emplace(k, id);
+  bc.push_back(bx);
+  bc.push_back(by);
+  bc.push_back(bz);
+  alloc.push_back(0);
+  nflag.resize(nflag.size() + 64, 0);
+  cflag.resize(cflag.size() + 64, 0);
+  return id;
+}
+
+void HostLevel::mark_page(int x, int y, int z) {
+  // a 4x4x8 page = two 4x4x4 blocks stacked in z
+  int bx = x >> 2, by = y >> 2, bz0 = (z >> 3) * 2;
+  int32_t b0 = get(bx, by, bz0);
+  alloc[b0] = 1;
+  int32_t b1 = get(bx, by, bz0 + 1);
+  alloc[b1] = 1;
+}
+
+static inline int loc(int x, int y, int z) { return ((x & 3) << 4) | ((y & 3) << 2) | (z & 3); }
+
+// Flag_Nodes (Flaging.h:127-142): cells of allocated blocks push their
+// ACTIVE / DIRICHLET bits (>>4) to their 8 nodes.
+static void flag_nodes(HostLevel& H) {
+  size_t nb0 = H.alloc.size();
+  for (size_t b = 0; b < nb0; ++b) {
+    if (!H.alloc[b]) continue;
+    int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
+    for (int l = 0; l < 64; ++l) {
+      uint16_t nf = (uint16_t)((H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL)) >> 4);
+      if (!nf) continue;
+      int x = bx * 4 + (l >> 4), y = by * 4 + ((l >> 2) & 3), z = bz * 4 + (l & 3);
+      for (int c = 0; c < 8; ++c) {
+        int X = x + ((c >> 2) & 1), Y = y + ((c >> 1) & 1), Z = z + (c & 1);
+        int32_t t = H.get(X >> 2, Y >> 2, Z >> 2);
+        H.nflag[(size_t)t * 64 + loc(X, Y, Z)] |= nf;
+      }
+    }
+  }
+}
+
+// Flag_Boundary_Nodes with band 3 (Flaging.h:155-184), evaluated as a
+// frontier expansion: in each pass an allocated ACTIVE node without Dirichlet
+// bits becomes BOUNDARY if one of its 26 neighbours carried DIRICHLET or
+// BOUNDARY bits at the start of the pass.
+static void flag_boundary(HostLevel& H) {
+  std::vector<std::array<int, 3>> seeds;
+  for (size_t b = 0; b < H.alloc.size(); ++b)
+    for (int l = 0; l < 64; ++l)
+      if (H.nflag[b * 64 + l] & (F_DIR_NODE | F_BOUNDARY))
+        seeds.push_back({H.bc[3 * b] * 4 + (l >> 4), H.bc[3 * b + 1] * 4 + ((l >> 2) & 3), H.bc[3 * b + 2] * 4 + (l & 3)});
+  for (int pass = 0; pass < 3; ++pass) {
+    std::vector<std::array<int, 3>> add;
+    std::unordered_set<uint64_t> seen;
+    for (auto& s : seeds)
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            int x = s[0] + dx, y = s[1] + dy, z = s[2] + dz;
+            int32_t b = H.find(x >> 2, y >> 2, z >> 2);
+            if (b < 0 || !H.alloc[b]) continue;
+            uint16_t f = H.nflag[(size_t)b * 64 + loc(x, y, z)];
+            if ((f & F_DIR_NODE) || !(f & F_ACTIVE_NODE) || (f & F_BOUNDARY)) continue;
+            uint64_t k = bkey(x, y, z);
+            if (!seen.insert(k).second) continue;
+            add.push_back({x, y, z});
+          }
+    for (auto& a : add) {
+      int32_t b = H.find(a[0] >> 2, a[1] >> 2, a[2] >> 2);
+      H.nflag[(size_t)b * 64 + loc(a[0], a[1], a[2])] |= F_BOUNDARY;
+    }
+    seeds = std::move(add);
+  }
+}
+
+static uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  auto spread = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+  };
+  return (spread(x) << 2) | (spread(y) << 1) | spread(z);
+}
+
+Layout build_layout(const ProblemDesc& p) {
+  Layout out;
+  if (p.nblocks <= 0) throw LayoutError("no density blocks", -1);
+  if (!(p.dx > 0)) throw LayoutError("dx must be positive", -1);
+  if (!(p.mu > 0) || p.lambda < 0) throw LayoutError("global_mu must be > 0 and global_lambda >= 0", -1);
+  int L = p.mg_levels;
+  out.use_mg = L != 0;
+  out.nlev = L == 0 ? 1 : L;
+  if (L < 0 || L > 12) throw LayoutError("mg_level out of range", -1);
+  if (out.nlev == 2)
+    throw LayoutError(
+        "mg_level=2 is not supported: level 1 is a pass-through level without a matrix "
+        "(Multigrid_Hierarchy.h:215-226); use 1 or >=3",
+        -1);
+  std::vector<HostLevel> H(out.nlev);
+
+  // ---- level 0: pages of the input blocks (SPGrid_CPU_Solver.cpp:107-138)
+  HostLevel& H0 = H[0];
+  H0.map.reserve((size_t)p.nblocks * 3);
+  std::unordered_set<uint64_t> pages;
+  for (int64_t b = 0; b < p.nblocks; ++b) {
+    const int32_t* bb = p.block_base + 3 * b;
+    H0.mark_page(bb[0], bb[1], bb[2]);
+    pages.insert(bkey(bb[0] >> 2, bb[1] >> 2, bb[2] >> 3));
+  }
+  out.n_allocated_pages = (int64_t)pages.size();
+  // padding check (Multigrid_Hierarchy.h:145-157)
+  for (uint64_t k : pages) {
+    int pc[3] = {(int)((k >> 42) & 0x1fffff) - (1 << 20), (int)((k >> 21) & 0x1fffff) - (1 << 20),
+                 (int)(k & 0x1fffff) - (1 << 20)};
+    for (int d = 0; d < 3; ++d) {
+      int o = pc[d] * kPage[d];
+      if (o < kB[d] || o >= p.res[d] - 2 * kB[d])
+        throw LayoutError("padding violated: page origin (" + std::to_string(pc[0] * 4) + "," +
+                              std::to_string(pc[1] * 4) + "," + std::to_string(pc[2] * 8) + ") outside [B, res-2B)",
+                          -2);
+    }
+  }
+  // cell moduli (SIMP) and ACTIVE_CELL (Flaging.h:93-106)
+  std::vector<double> E0(H0.alloc.size() * 64, 0.0), R0(H0.alloc.size() * 64, 0.0);
+  for (int64_t b = 0; b < p.nblocks; ++b) {
+    const int32_t* bb = p.block_base + 3 * b;
+    for (int l = 0; l < 64; ++l) {
+      int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
+      int32_t hb = H0.find(x >> 2, y >> 2, z >> 2);
+      if (hb < 0 || !H0.alloc[hb]) continue;  // cells outside set pages are dropped
+      double d = p.density[b * 64 + l];
+      if (std::isnan(d)) throw LayoutError("density contains NaN", -6);
+      size_t s = (size_t)hb * 64 + loc(x, y, z);
+      R0[s] = d;
+      if (d > 0) {
+        double e = stiffness_mapping(p.penalty, p.min_stiffness, p.min_density, d);
+        E0[s] = e;
+        if (e * p.mu != 0.0) H0.cflag[s] |= F_ACTIVE_CELL;
+      }
+    }
+  }
+  // block activation check (Validate_Blocks.h:46-69)
+  for (size_t b = 0; b < H0.alloc.size(); ++b)
+    for (int l = 0; l < 64; ++l) {
+      if (!(H0.cflag[b * 64 + l] & F_ACTIVE_CELL)) continue;
+      int x = H0.bc[3 * b] * 4 + (l >> 4), y = H0.bc[3 * b + 1] * 4 + ((l >> 2) & 3), z = H0.bc[3 * b + 2] * 4 + (l & 3);
+      for (int c = 1; c < 8; ++c)
+        if (!H0.node_allocated(x + ((c >> 2) & 1), y + ((c >> 1) & 1), z + (c & 1)))
+          throw LayoutError("block activation check failed at cell (" + std::to_string(x) + "," + std::to_string(y) +
+                                "," + std::to_string(z) + "): its nodes are not all in allocated blocks",
+                            -2);
+    }
+  // Dirichlet nodes -> Dirichlet cells (SPGrid_CPU_Solver.cpp:147-192)
+  {
+    std::unordered_set<uint64_t> dset[3];
+    for (int64_t i = 0; i < p.ndir; ++i) {
+      int v = p.dir_axis[i];
+      if (v < 0 || v > 2) throw LayoutError("Dirichlet axis out of range", -1);
+      dset[v].insert(bkey(p.dir_coord[3 * i], p.dir_coord[3 * i + 1], p.dir_coord[3 * i + 2]));
+    }
+    for (int64_t i = 0; i < p.ndir; ++i) {
+      int v = p.dir_axis[i];
+      int x = p.dir_coord[3 * i], y = p.dir_coord[3 * i + 1], z = p.dir_coord[3 * i + 2];
+      bool all = true;
+      for (int c = 1; c < 8 && all; ++c)
+        all = dset[v].count(bkey(x + ((c >> 2) & 1), y + ((c >> 1) & 1), z + (c & 1))) != 0;
+      if (!all) continue;
+      int32_t hb = H0.find(x >> 2, y >> 2, z >> 2);
+      if (hb < 0 || !H0.alloc[hb]) continue;
+      H0.cflag[(size_t)hb * 64 + loc(x, y, z)] |= (uint16_t)(F_DIR_CELL_X << v);
+    }
+  }
+  // ---- coarse levels (Multigrid_Hierarchy.h:158-195)
+  for (int l = 1; l < out.nlev; ++l) {
+    HostLevel& F = H[l - 1];
+    HostLevel& C = H[l];
+    size_t nbf = F.alloc.size();
+    for (size_t b = 0; b < nbf; ++b) {
+      if (!F.alloc[b]) continue;
+      for (int q = 0; q < 64; ++q) {
+        uint16_t cf = F.cflag[b * 64 + q];
+        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+        int x = F.bc[3 * b] * 4 + (q >> 4), y = F.bc[3 * b + 1] * 4 + ((q >> 2) & 3), z = F.bc[3 * b + 2] * 4 + (q & 3);
+        int X = (x + kB[0]) >> 1, Y = (y + kB[1]) >> 1, Z = (z + kB[2]) >> 1;
+        int32_t cb = C.get(X >> 2, Y >> 2, Z >> 2);
+        C.cflag[(size_t)cb * 64 + loc(X, Y, Z)] |= (uint16_t)(cf & (F_ACTIVE_CELL | F_DIR_CELL));
+        if (cf & F_ACTIVE_CELL)
+          for (int c = 0; c < 8; ++c) C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
+        if (cf & F_DIR_CELL)
+          for (int c = 1; c < 8; ++c) C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
+      }
+    }
+  }
+  for (int l = 0; l < out.nlev; ++l) flag_nodes(H[l]);
+  for (int l = 0; l < out.nlev; ++l) flag_boundary(H[l]);
+
+  // ---- GPU block sets: allocated blocks with an ACTIVE node, Morton order
+  std::vector<std::unordered_map<uint64_t, int32_t>> gmap(out.nlev);
+  out.lv.resize(out.nlev);
+  for (int l = 0; l < out.nlev; ++l) {
+    HostLevel& Hl = H[l];
+    std::vector<std::pair<uint64_t, int32_t>> sel;
+    for (size_t b = 0; b < Hl.alloc.size(); ++b) {
+      if (!Hl.alloc[b]) continue;
+      bool act = false;
+      for (int q = 0; q < 64 && !act; ++q) act = (Hl.nflag[b * 64 + q] & F_ACTIVE_NODE) != 0;
+      if (!act) continue;
+      sel.push_back({morton3((uint32_t)(Hl.bc[3 * b] + 4096), (uint32_t)(Hl.bc[3 * b + 1] + 4096),
+                             (uint32_t)(Hl.bc[3 * b + 2] + 4096)),
+                     (int32_t)b});
+    }
+    std::sort(sel.begin(), sel.end());
+    LevelDesc& D = out.lv[l];
+    D.nb = (int64_t)sel.size();
+    D.base.resize(D.nb * 3);
+    D.flags.assign(D.nb * 64, 0);
+    gmap[l].reserve(sel.size() * 2);
+    for (int64_t g = 0; g < D.nb; ++g) {
+      int32_t b = sel[g].second;
+      for (int d = 0; d < 3; ++d) D.base[3 * g + d] = Hl.bc[3 * b + d] * 4;
+      gmap[l].emplace(bkey(Hl.bc[3 * b], Hl.bc[3 * b + 1], Hl.bc[3 * b + 2]), (int32_t)g);
+      for (int q = 0; q < 64; ++q) {
+        uint16_t f = Hl.nflag[(size_t)b * 64 + q];
+        uint8_t df = 0;
+        if (f & F_ACTIVE_NODE) { df |= D_ACTIVE; D.n_active++; }
+        if (f & 0x20) df |= 0x02;
+        if (f & 0x40) df |= 0x04;
+        if (f & 0x80) df |= 0x08;
+        if (f & F_BOUNDARY) df |= D_BOUNDARY;
+        D.flags[g * 64 + q] = df;
+        if ((df & D_BOUNDARY)) {
+          int x = D.base[3 * g] + (q >> 4), y = D.base[3 * g + 1] + ((q >> 2) & 3), z = D.base[3 * g + 2] + (q & 3);
+          int colour = ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
+          D.bnd_list[colour].push_back((int32_t)(g * 64 + q));
+        }
+        if (df & D_DIR) D.dir_list.push_back((int32_t)(g * 64 + q));
+      }
+    }
+    if (D.nb == 0) throw LayoutError("level " + std::to_string(l) + " has no active nodes", -1);
+    auto gfind = [&](int bx, int by, int bz) {
+      auto it = gmap[l].find(bkey(bx, by, bz));
+      return it == gmap[l].end() ? -1 : it->second;
+    };
+    D.nbr.resize(D.nb * 27);
+    for (int64_t g = 0; g < D.nb; ++g) {
+      int bx = D.base[3 * g] >> 2, by = D.base[3 * g + 1] >> 2, bz = D.base[3 * g + 2] >> 2;
+      for (int s = 0; s < 27; ++s) D.nbr[g * 27 + s] = gfind(bx + s / 9 - 1, by + (s / 3) % 3 - 1, bz + s % 3 - 1);
+    }
+  }
+  // transfers
+  for (int l = 0; l < out.nlev; ++l) {
+    LevelDesc& D = out.lv[l];
+    if (l >= 1) {
+      D.restrict_src.resize(D.nb * 27);
+      for (int64_t g = 0; g < D.nb; ++g) {
+        int f0[3];
+        for (int d = 0; d < 3; ++d) f0[d] = (2 * D.base[3 * g + d] - kB[d]) >> 2;
+        for (int s = 0; s < 27; ++s) {
+          auto it = gmap[l - 1].find(bkey(f0[0] + s / 9 - 1, f0[1] + (s / 3) % 3 - 1, f0[2] + s % 3 - 1));
+          D.restrict_src[g * 27 + s] = it == gmap[l - 1].end() ? -1 : it->second;
+        }
+      }
+    }
+    if (l + 1 < out.nlev) {
+      D.prolong_src.resize(D.nb * 8);
+      for (int64_t g = 0; g < D.nb; ++g) {
+        int c0[3];
+        for (int d = 0; d < 3; ++d) c0[d] = ((D.base[3 * g + d] + kB[d]) >> 1) >> 2;
+        for (int s = 0; s < 8; ++s) {
+          auto it = gmap[l + 1].find(bkey(c0[0] + ((s >> 2) & 1), c0[1] + ((s >> 1) & 1), c0[2] + (s & 1)));
+          D.prolong_src[g * 8 + s] = it == gmap[l + 1].end() ? -1 : it->second;
+        }
+      }
+    }
+  }
+  // explicit levels (Multigrid_Hierarchy.h:215-226): cells and Galerkin tables
+  int first_explicit = out.nlev == 1 ? 0 : out.explicit_level;
+  if (out.use_mg)
+    for (int l = first_explicit; l < out.nlev; ++l) {
+      LevelDesc& D = out.lv[l];
+      HostLevel& Hl = H[l];
+      D.is_explicit = true;
+      D.cell_row.assign(D.nb * 64, -1);
+      int nper = l >= 2 ? (1 << (l - 2)) : 0;
+      for (int64_t g = 0; g < D.nb; ++g) {
+        int bx = D.base[3 * g] >> 2, by = D.base[3 * g + 1] >> 2, bz = D.base[3 * g + 2] >> 2;
+        int32_t hb = Hl.find(bx, by, bz);
+        for (int q = 0; q < 64; ++q) {
+          if (!(Hl.cflag[(size_t)hb * 64 + q] & F_ACTIVE_CELL)) continue;
+          D.cell_row[g * 64 + q] = (int32_t)D.ncell++;
+          D.cell_gpu_index.push_back((int32_t)(g * 64 + q));
+          if (l >= 2) {
+            int C[3] = {D.base[3 * g] + (q >> 4), D.base[3 * g + 1] + ((q >> 2) & 3), D.base[3 * g + 2] + (q & 3)};
+            int fb[3];
+            for (int d = 0; d < 3; ++d) fb[d] = ((1 << l) * C[d] - ((1 << l) - 1) * kB[d]) >> 2;
+            for (int i = 0; i < nper; ++i)
+              for (int j = 0; j < nper; ++j)
+                for (int k = 0; k < nper; ++k) {
+                  auto it = gmap[0].find(bkey(fb[0] + i, fb[1] + j, fb[2] + k));
+                  D.cell_fine_blocks.push_back(it == gmap[0].end() ? -1 : it->second);
+                }
+          }
+        }
+      }
+    }
+  // ---- level-0 cell data and input mapping
+  {
+    LevelDesc& D = out.lv[0];
+    out.E.assign(D.nb * 64, 0.0);
+    out.rho.assign(D.nb * 64, 0.0);
+    for (int64_t g = 0; g < D.nb; ++g) {
+      int32_t hb = H0.find(D.base[3 * g] >> 2, D.base[3 * g + 1] >> 2, D.base[3 * g + 2] >> 2);
+      for (int q = 0; q < 64; ++q) {
+        size_t s = (size_t)hb * 64 + q;
+        if (H0.cflag[s] & F_ACTIVE_CELL) { out.E[g * 64 + q] = E0[s]; out.n_active_cells++; }
+        out.rho[g * 64 + q] = R0[s];
+      }
+    }
+    out.nin_blocks = p.nblocks;
+    out.in_node_slot.resize(p.nblocks * 64);
+    for (int64_t b = 0; b < p.nblocks; ++b) {
+      const int32_t* bb = p.block_base + 3 * b;
+      for (int l = 0; l < 64; ++l) {
+        int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
+        auto it = gmap[0].find(bkey(x >> 2, y >> 2, z >> 2));
+        out.in_node_slot[b * 64 + l] = it == gmap[0].end() ? -1 : (int32_t)(it->second * 64 + loc(x, y, z));
+      }
+    }
+    // u_D (SPGrid_CPU_Solver.cpp:241-246,281-285): written at every listed node
+    std::unordered_map<uint64_t, std::vector<int64_t>> in_nodes_off;  // for nodes outside the GPU set
+    for (int64_t b = 0; b < p.nblocks; ++b)
+      for (int l = 0; l < 64; ++l)
+        if (out.in_node_slot[b * 64 + l] < 0) {
+          const int32_t* bb = p.block_base + 3 * b;
+          in_nodes_off[bkey(bb[0] + (l >> 4), bb[1] + ((l >> 2) & 3), bb[2] + (l & 3))].push_back(b * 64 + l);
+        }
+    for (int64_t i = 0; i < p.ndir; ++i) {
+      int x = p.dir_coord[3 * i], y = p.dir_coord[3 * i + 1], z = p.dir_coord[3 * i + 2];
+      auto it = gmap[0].find(bkey(x >> 2, y >> 2, z >> 2));
+      if (it != gmap[0].end()) {
+        out.ud_slot.push_back((int32_t)((it->second * 64 + loc(x, y, z)) * 3 + p.dir_axis[i]));
+        out.ud_value.push_back(p.dir_value[i]);
+      } else {
+        auto jt = in_nodes_off.find(bkey(x, y, z));
+        if (jt != in_nodes_off.end())
+          for (int64_t n : jt->second) {
+            out.ud_extra.push_back(n * 3 + p.dir_axis[i]);
+            out.ud_extra_value.push_back(p.dir_value[i]);
+          }
+      }
+    }
+  }
+  return out;
+}
+
+}  // namespace spg

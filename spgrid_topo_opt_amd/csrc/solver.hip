@@ -1,0 +1,714 @@
+// solver.hip -- see solver.hpp.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "solver.hpp"
+
+namespace spg {
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) throw SpgError(SPG_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define RBCHK(x)                                                                                           \
+  do {                                                                                                     \
+    rocblas_status s_ = (x);                                                                               \
+    if (s_ != rocblas_status_success)                                                                      \
+      throw SpgError(SPG_E_HIP, std::string(#x) + ": rocblas status " + std::to_string((int)s_));          \
+  } while (0)
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// a bottom level above this many unknowns is refused (dense FP64 inverse)
+static constexpr int64_t kMaxBottomDofs = 60000;
+
+DBuf::~DBuf() {
+  if (p) (void)hipFree(p);
+}
+void DBuf::alloc(size_t n) {
+  if (p) {
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  bytes = n;
+  if (n) HIPCHK(hipMalloc(&p, n));
+}
+
+template <class T>
+static void upload(DBuf& b, const std::vector<T>& v) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <> float* LevelRT::u<float>() const { return u32.as<float>(); }
+template <> double* LevelRT::u<double>() const { return u64.as<double>(); }
+template <> float* LevelRT::b<float>() const { return b32.as<float>(); }
+template <> double* LevelRT::b<double>() const { return b64.as<double>(); }
+template <> float* LevelRT::r<float>() const { return r32.as<float>(); }
+template <> double* LevelRT::r<double>() const { return r64.as<double>(); }
+template <> const float* LevelRT::st<float>() const { return st32.as<float>(); }
+template <> const double* LevelRT::st<double>() const { return st64.as<double>(); }
+template <> const float* Solver::E<float>() const { return E32_.as<float>(); }
+template <> const double* Solver::E<double>() const { return E64_.as<double>(); }
+
+Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
+  const double t0 = now_s();
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw SpgError(SPG_E_HIP, "no HIP device available");
+  if (cfg.device < 0 || cfg.device >= ndev) throw SpgError(SPG_E_INVALID, "device ordinal out of range");
+  HIPCHK(hipSetDevice(cfg.device));
+  size_t free0 = 0, total0 = 0;
+  HIPCHK(hipMemGetInfo(&free0, &total0));
+  HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  RBCHK(rocblas_create_handle(&rb_));
+  RBCHK(rocblas_set_stream(rb_, st_));
+  lay_ = build_layout(pd);
+  double kmu[576], kla[576];
+  element_tables(kmu, kla);
+  for (int e = 0; e < 576; ++e) K0_[e] = cfg.mu * kmu[e] + cfg.lambda * kla[e];
+  upload_k0(K0_, st_);
+  upload_layout();
+  build_operators();
+  sync();
+  size_t free1 = 0;
+  HIPCHK(hipMemGetInfo(&free1, &total0));
+  dev_bytes_ = (double)free0 - (double)free1;
+  setup_s_ = now_s() - t0;
+}
+
+Solver::~Solver() {
+  if (rb_) rocblas_destroy_handle(rb_);
+  if (hscal_) (void)hipHostFree(hscal_);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Solver::sync() { HIPCHK(hipStreamSynchronize(st_)); }
+
+void Solver::upload_layout() {
+  const int L = lay_.nlev;
+  lv_.clear();
+  lv_.resize(L);
+  for (int l = 0; l < L; ++l) {
+    const LevelDesc& D = lay_.lv[l];
+    LevelRT& R = lv_[l];
+    R.d.nb = (int)D.nb;
+    R.n3 = D.nb * 192;
+    upload(R.nbr, D.nbr);
+    upload(R.flags, D.flags);
+    upload(R.base, D.base);
+    if (!D.restrict_src.empty()) upload(R.rsrc, D.restrict_src);
+    if (!D.prolong_src.empty()) upload(R.psrc, D.prolong_src);
+    for (int c = 0; c < 8; ++c) {
+      if (!D.bnd_list[c].empty()) upload(R.bnd[c], D.bnd_list[c]);
+      R.d.bnd[c] = R.bnd[c].as<int>();
+      R.d.nbnd[c] = (int)D.bnd_list[c].size();
+    }
+    R.d.nbr = R.nbr.as<int>();
+    R.d.flags = R.flags.as<uint8_t>();
+    R.d.base = R.base.as<int>();
+    R.d.rsrc = R.rsrc.as<int>();
+    R.d.psrc = R.psrc.as<int>();
+    R.u32.alloc(R.n3 * 4);
+    R.b32.alloc(R.n3 * 4);
+    R.r32.alloc(R.n3 * 4);
+    R.u64.alloc(R.n3 * 8);
+    R.b64.alloc(R.n3 * 8);
+    R.r64.alloc(R.n3 * 8);
+  }
+  // level-0 cell data
+  upload(E64_, lay_.E);
+  std::vector<float> e32(lay_.E.begin(), lay_.E.end());
+  upload(E32_, e32);
+  upload(rho_, lay_.rho);
+  upload(in_slot_, lay_.in_node_slot);
+  upload(ud_idx_, lay_.ud_slot);
+  upload(ud_val_, lay_.ud_value);
+  n_ud_ = (int64_t)lay_.ud_slot.size();
+  const int64_t n3 = lv_[0].n3;
+  for (DBuf* b : {&x32_, &rr32_, &s32_, &q32_, &z32_}) b->alloc(n3 * 4);
+  for (DBuf* b : {&x64_, &rr64_, &s64_, &q64_, &z64_, &u_, &f_, &res_, &tmp_}) b->alloc(n3 * 8);
+  scal_.alloc(sizeof(CgScalars) + reduce_partials(0) * sizeof(double));
+  part_.alloc((reduce_partials(0) + 16) * sizeof(double));
+  HIPCHK(hipHostMalloc((void**)&hscal_, 256));
+}
+
+void Solver::build_operators() {
+  const int L = lay_.nlev;
+  if (!lay_.use_mg) return;
+  const double one = 1.0, zero = 0.0;
+  for (int l = 0; l < L; ++l) {
+    const LevelDesc& D = lay_.lv[l];
+    if (!D.is_explicit) continue;
+    LevelRT& R = lv_[l];
+    const int depth = (L == 1) ? 0 : l;
+    const int64_t npos = (int64_t)1 << (3 * depth);
+    upload(R.cell_row, D.cell_row);
+    if (!D.cell_fine_blocks.empty()) upload(R.cfb, D.cell_fine_blocks);
+    upload(R.cgi, D.cell_gpu_index);
+    if ((int)posmat_cache_.size() <= depth) posmat_cache_.resize(depth + 1);
+    if (posmat_cache_[depth].empty()) posmat_cache_[depth] = galerkin_position_matrices(depth, K0_);
+    DBuf dM, G, Cm;
+    upload(dM, posmat_cache_[depth]);
+    const int64_t ncell = D.ncell;
+    R.st64.alloc((size_t)D.nb * 243 * 64 * 8);
+    R.st32.alloc((size_t)D.nb * 243 * 64 * 4);
+    if (ncell > 0) {
+      G.alloc((size_t)(ncell * npos) * 8);
+      Cm.alloc((size_t)ncell * 576 * 8);
+      launch_galerkin_gather(st_, ncell, depth, R.cfb.as<int>(), R.cgi.as<int>(), E64_.as<double>(), G.as<double>());
+      // Cm^T (576 x ncell) = M^T (576 x npos) * G^T (npos x ncell), column-major views
+      RBCHK(rocblas_dgemm(rb_, rocblas_operation_none, rocblas_operation_none, 576, (rocblas_int)ncell,
+                          (rocblas_int)npos, &one, dM.as<double>(), 576, G.as<double>(), (rocblas_int)npos, &zero,
+                          Cm.as<double>(), 576));
+    }
+    launch_galerkin_assemble(st_, R.d, R.cell_row.as<int>(), Cm.as<double>(), cfg_.dx, R.st64.as<double>());
+    launch_stencil_to_float(st_, R.st64.as<double>(), R.st32.as<float>(), (int64_t)D.nb * 243 * 64);
+    sync();
+  }
+  // bottom: FP64 Cholesky of the Dirichlet-deleted stencil matrix, then its inverse
+  // (Multigrid_Hierarchy.h:332-341; Elasticity_Matrix.cpp:15-233 factorizes with PARDISO)
+  const int B = L - 1;
+  const LevelDesc& D = lay_.lv[B];
+  std::vector<int32_t> var(D.nb * 64, -1);
+  int64_t nv = 0;
+  for (int64_t s = 0; s < D.nb * 64; ++s)
+    if (D.flags[s] & D_ACTIVE) {
+      var[s] = (int32_t)nv;
+      nv += 3;
+    }
+  nvar_ = nv;
+  if (nv > kMaxBottomDofs)
+    throw SpgError(SPG_E_INVALID, "bottom level has " + std::to_string(nv) + " unknowns (limit " +
+                                      std::to_string(kMaxBottomDofs) + "); increase mg_level");
+  upload(var_of_, var);
+  Ainv_.alloc((size_t)nv * nv * 8);
+  xb_.alloc(nv * 8);
+  yb_.alloc(nv * 8);
+  info_.alloc(16);
+  HIPCHK(hipMemsetAsync(Ainv_.p, 0, (size_t)nv * nv * 8, st_));
+  launch_dense_assemble(st_, lv_[B].d, lv_[B].st64.as<double>(), var_of_.as<int>(), nv, Ainv_.as<double>());
+  RBCHK(rocsolver_dpotrf(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv_.as<double>(), (rocblas_int)nv,
+                         info_.as<rocblas_int>()));
+  int hinfo = 0;
+  HIPCHK(hipMemcpyAsync(&hinfo, info_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+  sync();
+  if (hinfo != 0)
+    throw SpgError(SPG_E_SINGULAR, "bottom Cholesky failed: leading minor " + std::to_string(hinfo) +
+                                       " not positive definite");
+  RBCHK(rocsolver_dpotri(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv_.as<double>(), (rocblas_int)nv,
+                         info_.as<rocblas_int>()));
+  launch_symmetrize_lower(st_, Ainv_.as<double>(), nv);
+  sync();
+}
+
+void Solver::update_density(const double* density) {
+  // recompute E (SIMP) for the existing cells, then the coarse operators
+  const Layout& Lc = lay_;
+  std::vector<double> E(Lc.E.size(), 0.0), rho(Lc.rho.size(), 0.0);
+  for (int64_t b = 0; b < Lc.nin_blocks; ++b)
+    for (int l = 0; l < 64; ++l) {
+      const int32_t s = Lc.in_node_slot[b * 64 + l];
+      if (s < 0) continue;
+      const double d = density[b * 64 + l];
+      if (std::isnan(d)) throw SpgError(SPG_E_NAN, "density contains NaN");
+      rho[s] = d;
+      if (Lc.E[s] != 0.0) {
+        if (!(d > 0)) throw SpgError(SPG_E_INVALID, "density update would deactivate a cell (topology change)");
+        E[s] = stiffness_mapping(cfg_.penalty, cfg_.min_stiffness, cfg_.min_density, d);
+      } else if (d > 0) {
+        throw SpgError(SPG_E_INVALID, "density update would activate a cell (topology change)");
+      }
+    }
+  lay_.E = E;
+  lay_.rho = rho;
+  HIPCHK(hipMemcpy(E64_.p, E.data(), E.size() * 8, hipMemcpyHostToDevice));
+  std::vector<float> e32(E.begin(), E.end());
+  HIPCHK(hipMemcpy(E32_.p, e32.data(), e32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(rho_.p, rho.data(), rho.size() * 8, hipMemcpyHostToDevice));
+  build_operators();
+}
+
+void Solver::set_forces(int64_t n, const int32_t* coord, const double* value) {
+  // ForceOnNode assignment (SPGrid_CPU_Solver.cpp:289-296): later entries win
+  if (gmap0_.empty()) {
+    const LevelDesc& D = lay_.lv[0];
+    gmap0_.reserve(D.nb * 2);
+    for (int64_t b = 0; b < D.nb; ++b)
+      gmap0_.emplace(bkey(D.base[3 * b] >> 2, D.base[3 * b + 1] >> 2, D.base[3 * b + 2] >> 2), (int32_t)b);
+  }
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+  for (int64_t i = 0; i < n; ++i) {
+    const int x = coord[3 * i], y = coord[3 * i + 1], z = coord[3 * i + 2];
+    auto it = gmap0_.find(bkey(x >> 2, y >> 2, z >> 2));
+    if (it == gmap0_.end()) continue;  // forces outside the solved blocks are ignored
+    const int32_t g = it->second;
+    const int q = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+    for (int c = 0; c < 3; ++c) {
+      idx.push_back((g * 64 + q) * 3 + c);
+      val.push_back(value[3 * i + c]);
+    }
+  }
+  upload(f_idx_, idx);
+  upload(f_val_, val);
+  n_f_ = (int64_t)idx.size();
+}
+
+double Solver::read_scalar(const double* dev) {
+  HIPCHK(hipMemcpyAsync(hscal_, dev, sizeof(double), hipMemcpyDeviceToHost, st_));
+  sync();
+  return hscal_[0];
+}
+
+// ------------------------------------------------------------------ V-cycle
+template <class T>
+void Solver::smooth(int l, T* u, const T* b, bool forward) {
+  static const int order_st[2][8] = {{1, 5, 4, 6, 7, 3, 2, 0}, {0, 2, 3, 7, 6, 4, 5, 1}};
+  const LevelRT& R = lv_[l];
+  auto sweep = [&](bool boundary) {
+    for (int c = 0; c < 8; ++c) {
+      if (l == 0) {
+        const int colour = forward ? c : 7 - c;
+        launch_gs0<T>(st_, R.d, E<T>(), dxT<T>(), u, b, colour, boundary);
+      } else {
+        const int colour = order_st[forward ? 0 : 1][c];
+        launch_st_gs<T>(st_, R.d, R.st<T>(), u, b, colour, boundary);
+      }
+    }
+  };
+  // Multigrid_Hierarchy.h:577-586 / 605-617
+  for (int i = 0; i < cfg_.boundary_iters; ++i) sweep(true);
+  for (int i = 0; i < cfg_.smooth_iters; ++i) sweep(false);
+  for (int i = 0; i < cfg_.boundary_iters; ++i) sweep(true);
+}
+
+template <class T>
+void Solver::level_residual(int l, const T* u, const T* b, T* r) {
+  if (l == 0) launch_apply<T>(st_, lv_[0].d, E<T>(), dxT<T>(), u, b, r, 1);
+  else launch_st_residual<T>(st_, lv_[l].d, lv_[l].st<T>(), u, b, r);
+}
+
+template <class T>
+void Solver::bottom(const T* b, T* u) {
+  const LevelRT& R = lv_[lay_.nlev - 1];
+  const double one = 1.0, zero = 0.0;
+  launch_bottom_gather<T>(st_, R.d, var_of_.as<int>(), b, xb_.as<double>());
+  RBCHK(rocblas_dgemv(rb_, rocblas_operation_none, (rocblas_int)nvar_, (rocblas_int)nvar_, &one, Ainv_.as<double>(),
+                      (rocblas_int)nvar_, xb_.as<double>(), 1, &zero, yb_.as<double>(), 1));
+  launch_bottom_scatter<T>(st_, R.d, var_of_.as<int>(), yb_.as<double>(), b, u);
+}
+
+template <class T>
+void Solver::vcycle(const T* b0, T* z) {
+  const int L = lay_.nlev;
+  const int ex = lay_.explicit_level;
+  if (!lay_.use_mg) {
+    launch_copy<T>(st_, b0, z, lv_[0].n3);
+    return;
+  }
+  if (L == 1) {
+    bottom<T>(b0, z);
+    return;
+  }
+  launch_zero<T>(st_, z, lv_[0].n3);
+  for (int l = 0; l < L - 1; ++l) {
+    T* u = l == 0 ? z : lv_[l].u<T>();
+    const T* b = l == 0 ? b0 : lv_[l].b<T>();
+    const bool sm = (l == 0 || l >= ex);
+    if (sm) smooth<T>(l, u, b, true);
+    launch_zero<T>(st_, lv_[l + 1].u<T>(), lv_[l + 1].n3);
+    if (sm) {
+      level_residual<T>(l, u, b, lv_[l].r<T>());
+      launch_restrict<T>(st_, lv_[l + 1].d, lv_[l].r<T>(), lv_[l + 1].b<T>());
+    } else {
+      launch_restrict<T>(st_, lv_[l + 1].d, b, lv_[l + 1].b<T>());
+    }
+  }
+  bottom<T>(lv_[L - 1].b<T>(), lv_[L - 1].u<T>());
+  for (int l = L - 2; l >= 0; --l) {
+    T* u = l == 0 ? z : lv_[l].u<T>();
+    const T* b = l == 0 ? b0 : lv_[l].b<T>();
+    launch_prolongate<T>(st_, lv_[l].d, lv_[l + 1].u<T>(), u);
+    if (l == 0 || l >= ex) smooth<T>(l, u, b, false);
+  }
+}
+
+// ---------------------------------------------------------------------- PCG
+// CONJUGATE_GRADIENT_4Ch::Solve (CG/CONJUGATE_GRADIENT.cpp:21-61): projected
+// PCG on x (starting at 0) with the L_inf stop test; scalars stay FP64 on the
+// device, the host reads ||r||_inf once per iteration.
+template <class T>
+int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm) {
+  const LevelRT& R0 = lv_[0];
+  const int64_t n = R0.n3;
+  T* s = sizeof(T) == 4 ? (T*)s32_.p : (T*)s64_.p;
+  T* q = sizeof(T) == 4 ? (T*)q32_.p : (T*)q64_.p;
+  T* z = sizeof(T) == 4 ? (T*)z32_.p : (T*)z64_.p;
+  CgScalars* sc = scal_.as<CgScalars>();
+  double* part = part_.as<double>();
+  launch_zero<T>(st_, x, n);
+  launch_zero<T>(st_, s, n);
+  launch_project<T>(st_, R0.d, r, true);
+  launch_cg_scalars(st_, sc, 3);
+  launch_linf<T>(st_, r, n, part, &sc->nrm);
+  int it = 0;
+  int rc = SPG_E_NOT_CONVERGED;
+  for (;; ++it) {
+    HIPCHK(hipMemcpyAsync(hscal_, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st_));
+    sync();
+    const CgScalars* h = reinterpret_cast<const CgScalars*>(hscal_);
+    *last_nrm = h->nrm;
+    if (h->singular) {
+      rc = SPG_E_SINGULAR;
+      break;
+    }
+    if (std::isnan(h->nrm)) {
+      rc = SPG_E_NAN;
+      break;
+    }
+    if (h->nrm <= tol) {
+      rc = SPG_OK;
+      break;
+    }
+    if (it == maxit) break;
+    vcycle<T>(r, z);
+    launch_project<T>(st_, R0.d, z, true);
+    launch_dot<T>(st_, z, r, n, part, &sc->rho);
+    launch_cg_scalars(st_, sc, 0);
+    launch_cg_direction<T>(st_, s, z, sc, n);
+    launch_apply<T>(st_, R0.d, E<T>(), dxT<T>(), s, nullptr, q, 2);
+    launch_dot<T>(st_, s, q, n, part, &sc->sq);
+    launch_cg_scalars(st_, sc, 1);
+    launch_cg_update<T>(st_, x, r, s, q, sc, n);
+    launch_cg_scalars(st_, sc, 2);
+  }
+  *iters = it;
+  return rc;
+}
+
+int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool) {
+  const LevelRT& R0 = lv_[0];
+  const int64_t n = R0.n3;
+  const double t0 = now_s();
+  double* part = part_.as<double>();
+  double* red = part + reduce_partials(0);
+  launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, u, f, res_.as<double>(), 1);
+  launch_linf<double>(st_, res_.as<double>(), n, part, red);
+  const double r0 = read_scalar(red);
+  const double target = prm.tolerance * r0;
+  int total = 0, passes = 0, rc = SPG_OK;
+  double rn = r0;
+  bool conv = (r0 == 0.0);
+  if (std::isnan(r0)) rc = SPG_E_NAN;
+  const int maxp = prm.max_passes < 1 ? 1 : prm.max_passes;
+  for (int pass = 0; pass < maxp && !conv && rc == SPG_OK; ++pass) {
+    double tolp = target;
+    if (prm.inner_tolerance > 0) tolp = std::max(prm.inner_tolerance * rn, 0.7 * target);
+    int it = 0;
+    double last = 0;
+    int prc;
+    if (prm.precision == 1) {
+      launch_copy<double>(st_, res_.as<double>(), rr64_.as<double>(), n);
+      prc = pcg<double>(x64_.as<double>(), rr64_.as<double>(), tolp, prm.max_iterations, &it, &last);
+      launch_axpy_d<double>(st_, u, x64_.as<double>(), n);
+    } else {
+      launch_cast_d2f(st_, res_.as<double>(), rr32_.as<float>(), n);
+      prc = pcg<float>(x32_.as<float>(), rr32_.as<float>(), tolp, prm.max_iterations, &it, &last);
+      launch_axpy_d<float>(st_, u, x32_.as<float>(), n);
+    }
+    total += it;
+    passes++;
+    launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, u, f, res_.as<double>(), 1);
+    launch_linf<double>(st_, res_.as<double>(), n, part, red);
+    rn = read_scalar(red);
+    if (prc == SPG_E_SINGULAR || prc == SPG_E_NAN) rc = prc;
+    if (std::isnan(rn)) rc = SPG_E_NAN;
+    if (rn <= target) conv = true;
+  }
+  launch_sumsq(st_, res_.as<double>(), n, part, red);
+  const double l2 = std::sqrt(read_scalar(red));
+  if (info) {
+    info->iterations = total;
+    info->passes = passes;
+    info->converged = conv ? 1 : 0;
+    info->levels = lay_.nlev;
+    info->r0_linf = r0;
+    info->r_linf = rn;
+    info->r_l2 = l2;
+    info->setup_seconds = setup_s_;
+    info->solve_seconds = now_s() - t0;
+    info->active_nodes = lay_.lv[0].n_active;
+    info->active_cells = lay_.n_active_cells;
+    info->gpu_blocks = lay_.lv[0].nb;
+    info->bottom_dofs = nvar_;
+    info->hbm_bytes = dev_bytes_;
+  }
+  if (rc == SPG_OK && !conv) rc = SPG_E_NOT_CONVERGED;
+  return rc;
+}
+
+int Solver::solve(const spg_solve_params& prm, spg_info* info) {
+  const LevelRT& R0 = lv_[0];
+  const int64_t n = R0.n3;
+  // u = u_D at the listed Dirichlet nodes (SPGrid_CPU_Solver.cpp:281-285)
+  launch_zero<double>(st_, u_.as<double>(), n);
+  launch_set_entries(st_, u_.as<double>(), ud_idx_.as<int>(), ud_val_.as<double>(), n_ud_);
+  // f = P P_null forces (SPGrid_CPU_Solver.cpp:289-299)
+  launch_zero<double>(st_, f_.as<double>(), n);
+  launch_set_entries(st_, f_.as<double>(), f_idx_.as<int>(), f_val_.as<double>(), n_f_);
+  launch_project<double>(st_, R0.d, f_.as<double>(), true);
+  const int rc = solve_rhs(prm, u_.as<double>(), f_.as<double>(), info, true);
+  have_solution_ = true;
+  return rc;
+}
+
+int Solver::solve_adjoint(const spg_solve_params& prm, const double* g_in3, double* lam_in3, spg_info* info) {
+  const LevelRT& R0 = lv_[0];
+  const int64_t n = R0.n3;
+  const int64_t nin = lay_.nin_blocks * 64;
+  launch_zero<double>(st_, tmp_.as<double>(), n);
+  launch_scatter_input_vec(st_, in_slot_.as<int>(), nin, g_in3, tmp_.as<double>());
+  launch_project<double>(st_, R0.d, tmp_.as<double>(), true);
+  // lambda with homogeneous Dirichlet data; reuse z64 as its storage
+  launch_zero<double>(st_, z64_.as<double>(), n);
+  DBuf lam;
+  lam.alloc(n * 8);
+  launch_zero<double>(st_, lam.as<double>(), n);
+  const int rc = solve_rhs(prm, lam.as<double>(), tmp_.as<double>(), info, false);
+  launch_gather_input_vec(st_, in_slot_.as<int>(), nin, lam.as<double>(), lam_in3);
+  sync();
+  return rc;
+}
+
+void Solver::pack_output(double* out4) const {
+  const int64_t n = lv_[0].n3;
+  std::vector<double> u(n), r(n);
+  HIPCHK(hipMemcpy(u.data(), u_.p, n * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(r.data(), res_.p, n * 8, hipMemcpyDeviceToHost));
+  const int64_t nin = lay_.nin_blocks * 64;
+  for (int64_t i = 0; i < nin; ++i) {
+    const int32_t s = lay_.in_node_slot[i];
+    double* o = out4 + i * 4;
+    if (s >= 0) {
+      const int blk = s >> 6, l = s & 63;
+      double m = 0;
+      for (int c = 0; c < 3; ++c) {
+        o[c] = u[(blk * 3 + c) * 64 + l];
+        m = std::max(m, std::fabs(r[(blk * 3 + c) * 64 + l]));
+      }
+      o[3] = m;
+    } else {
+      o[0] = o[1] = o[2] = o[3] = 0.0;
+    }
+  }
+  for (size_t k = 0; k < lay_.ud_extra.size(); ++k) {
+    const int64_t e = lay_.ud_extra[k];
+    out4[(e / 3) * 4 + (e % 3)] = lay_.ud_extra_value[k];
+  }
+}
+
+void Solver::displacement_in3(double* dev) const {
+  launch_gather_input_vec(st_, in_slot_.as<int>(), lay_.nin_blocks * 64, u_.as<double>(), dev);
+  HIPCHK(hipStreamSynchronize(st_));
+}
+
+double Solver::sensitivity(const double* u_in3_dev, const double* lam_in3_dev, double* dc_in_dev) {
+  const LevelRT& R0 = lv_[0];
+  const int64_t nslot = (int64_t)R0.d.nb * 64;
+  const int64_t nin = lay_.nin_blocks * 64;
+  DBuf dc, obj, uu;
+  dc.alloc(nslot * 8);
+  obj.alloc(nslot * 8);
+  const double* u = u_.as<double>();
+  if (u_in3_dev) {
+    uu.alloc(R0.n3 * 8);
+    launch_zero<double>(st_, uu.as<double>(), R0.n3);
+    launch_scatter_input_vec(st_, in_slot_.as<int>(), nin, u_in3_dev, uu.as<double>());
+    u = uu.as<double>();
+  }
+  const double* a = u;
+  if (lam_in3_dev) {
+    launch_zero<double>(st_, tmp_.as<double>(), R0.n3);
+    launch_scatter_input_vec(st_, in_slot_.as<int>(), nin, lam_in3_dev, tmp_.as<double>());
+    a = tmp_.as<double>();
+  }
+  launch_sensitivity(st_, R0.d, rho_.as<double>(), u, a, cfg_.penalty, cfg_.min_stiffness, dc.as<double>(),
+                     lam_in3_dev ? nullptr : obj.as<double>());
+  launch_gather_input_scalar(st_, in_slot_.as<int>(), nin, dc.as<double>(), dc_in_dev);
+  double objective = 0.0;
+  if (!lam_in3_dev) {
+    // ordered FP64 sum of the per-cell objective terms on the host
+    std::vector<double> h(nslot);
+    HIPCHK(hipMemcpyAsync(h.data(), obj.p, nslot * 8, hipMemcpyDeviceToHost, st_));
+    sync();
+    for (double v : h) objective += v;
+  }
+  sync();
+  return objective;
+}
+
+// ----------------------------------------------------------- kernel hooks
+template <class T>
+static std::vector<T> to_dev_layout(const double* in, int64_t nb) {
+  std::vector<T> v(nb * 192);
+  for (int64_t b = 0; b < nb; ++b)
+    for (int l = 0; l < 64; ++l)
+      for (int c = 0; c < 3; ++c) v[(b * 3 + c) * 64 + l] = (T)in[(b * 64 + l) * 3 + c];
+  return v;
+}
+template <class T>
+static void from_dev_layout(const std::vector<T>& v, int64_t nb, double* out) {
+  for (int64_t b = 0; b < nb; ++b)
+    for (int l = 0; l < 64; ++l)
+      for (int c = 0; c < 3; ++c) out[(b * 64 + l) * 3 + c] = (double)v[(b * 3 + c) * 64 + l];
+}
+
+void Solver::run_kernel(int op, int level, int precision, int forward, const double* in, const double* b,
+                        double* out) {
+  if (level < 0 || level >= lay_.nlev) throw SpgError(SPG_E_INVALID, "level out of range");
+  auto body = [&](auto tag) {
+    using T = decltype(tag);
+    auto put = [&](DBuf& d, const double* h, int64_t nb) {
+      std::vector<T> v = to_dev_layout<T>(h, nb);
+      d.alloc(v.size() * sizeof(T));
+      HIPCHK(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    };
+    auto get = [&](const T* d, int64_t nb) {
+      std::vector<T> v(nb * 192);
+      sync();
+      HIPCHK(hipMemcpy(v.data(), d, v.size() * sizeof(T), hipMemcpyDeviceToHost));
+      from_dev_layout<T>(v, nb, out);
+    };
+    const int64_t nb = lv_[level].d.nb;
+    DBuf A, Bv, O;
+    switch (op) {
+      case SPG_OP_APPLY:
+      case SPG_OP_RESIDUAL: {
+        if (level != 0) throw SpgError(SPG_E_INVALID, "apply is a level-0 operator");
+        put(A, in, nb);
+        if (op == SPG_OP_RESIDUAL) put(Bv, b, nb);
+        O.alloc(nb * 192 * sizeof(T));
+        launch_apply<T>(st_, lv_[0].d, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), O.as<T>(),
+                        op == SPG_OP_APPLY ? 0 : 1);
+        get(O.as<T>(), nb);
+        break;
+      }
+      case SPG_OP_GS_SWEEP:
+      case SPG_OP_GS_BOUNDARY: {
+        if (level != 0 && !lay_.lv[level].is_explicit) throw SpgError(SPG_E_INVALID, "level has no smoother");
+        put(A, in, nb);
+        put(Bv, b, nb);
+        static const int order_st[2][8] = {{1, 5, 4, 6, 7, 3, 2, 0}, {0, 2, 3, 7, 6, 4, 5, 1}};
+        for (int c = 0; c < 8; ++c) {
+          if (level == 0)
+            launch_gs0<T>(st_, lv_[0].d, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), forward ? c : 7 - c,
+                          op == SPG_OP_GS_BOUNDARY);
+          else
+            launch_st_gs<T>(st_, lv_[level].d, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(),
+                            order_st[forward ? 0 : 1][c], op == SPG_OP_GS_BOUNDARY);
+        }
+        get(A.as<T>(), nb);
+        break;
+      }
+      case SPG_OP_RESTRICT: {
+        if (level < 1) throw SpgError(SPG_E_INVALID, "restriction targets level >= 1");
+        put(A, in, lv_[level - 1].d.nb);
+        O.alloc(nb * 192 * sizeof(T));
+        launch_restrict<T>(st_, lv_[level].d, A.as<T>(), O.as<T>());
+        get(O.as<T>(), nb);
+        break;
+      }
+      case SPG_OP_PROLONGATE: {
+        if (level + 1 >= lay_.nlev) throw SpgError(SPG_E_INVALID, "prolongation needs a coarser level");
+        put(A, in, lv_[level + 1].d.nb);
+        put(Bv, b, nb);
+        launch_prolongate<T>(st_, lv_[level].d, A.as<T>(), Bv.as<T>());
+        get(Bv.as<T>(), nb);
+        break;
+      }
+      case SPG_OP_STENCIL_RES: {
+        if (!lay_.lv[level].is_explicit) throw SpgError(SPG_E_INVALID, "level has no stencils");
+        put(A, in, nb);
+        put(Bv, b, nb);
+        O.alloc(nb * 192 * sizeof(T));
+        launch_st_residual<T>(st_, lv_[level].d, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), O.as<T>());
+        get(O.as<T>(), nb);
+        break;
+      }
+      case SPG_OP_VCYCLE: {
+        put(Bv, b, lv_[0].d.nb);
+        O.alloc(lv_[0].n3 * sizeof(T));
+        vcycle<T>(Bv.as<T>(), O.as<T>());
+        get(O.as<T>(), lv_[0].d.nb);
+        break;
+      }
+      case SPG_OP_BOTTOM: {
+        const int B = lay_.nlev - 1;
+        if (level != B) throw SpgError(SPG_E_INVALID, "bottom solve lives on the last level");
+        put(Bv, b, nb);
+        O.alloc(nb * 192 * sizeof(T));
+        bottom<T>(Bv.as<T>(), O.as<T>());
+        get(O.as<T>(), nb);
+        break;
+      }
+      default:
+        throw SpgError(SPG_E_INVALID, "unknown op");
+    }
+  };
+  if (precision == 1) body(double());
+  else body(float());
+}
+
+void Solver::level_stencils(int level, double* out) {
+  if (level < 0 || level >= lay_.nlev || !lay_.lv[level].is_explicit)
+    throw SpgError(SPG_E_INVALID, "level has no stencils");
+  const int64_t nb = lv_[level].d.nb;
+  std::vector<double> v(nb * 243 * 64);
+  sync();
+  HIPCHK(hipMemcpy(v.data(), lv_[level].st64.p, v.size() * 8, hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < nb; ++b)
+    for (int e = 0; e < 243; ++e)
+      for (int l = 0; l < 64; ++l) out[(b * 64 + l) * 243 + e] = v[(b * 243 + e) * 64 + l];
+}
+
+double Solver::bench_apply(int precision, int reps) {
+  const LevelRT& R0 = lv_[0];
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  float ms = 0.f;
+  if (precision == 1) {
+    launch_copy<double>(st_, u_.as<double>(), x64_.as<double>(), R0.n3);
+    launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, x64_.as<double>(), nullptr, q64_.as<double>(), 2);
+    HIPCHK(hipEventRecord(a, st_));
+    for (int i = 0; i < reps; ++i)
+      launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, x64_.as<double>(), nullptr, q64_.as<double>(), 2);
+    HIPCHK(hipEventRecord(b, st_));
+  } else {
+    launch_cast_d2f(st_, u_.as<double>(), x32_.as<float>(), R0.n3);
+    launch_apply<float>(st_, R0.d, E<float>(), (float)cfg_.dx, x32_.as<float>(), nullptr, q32_.as<float>(), 2);
+    HIPCHK(hipEventRecord(a, st_));
+    for (int i = 0; i < reps; ++i)
+      launch_apply<float>(st_, R0.d, E<float>(), (float)cfg_.dx, x32_.as<float>(), nullptr, q32_.as<float>(), 2);
+    HIPCHK(hipEventRecord(b, st_));
+  }
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return (double)ms * 1e-3 / reps;
+}
+
+template void Solver::vcycle<float>(const float*, float*);
+template void Solver::vcycle<double>(const double*, double*);
+
+}  // namespace spg

@@ -1,0 +1,121 @@
+// solver.hpp -- device-resident narrow-band elasticity solver.
+//
+// Owns the level hierarchy on one GPU, the Galerkin stencils (assembled in
+// FP64, used in FP32 by the mixed inner solve), the FP64 bottom Cholesky
+// inverse, and the solution state; implements the reference's solve
+// (solver/SPGrid_CPU_Solver.cpp:241-341) with an iterated FP64
+// defect-correction outer loop around a FP32 (or FP64) MG-PCG.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <memory>
+#include <unordered_map>
+#include <string>
+#include <vector>
+
+#include "../../include/spgrid_topo_opt_amd.h"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace spg {
+
+struct SpgError : std::runtime_error {
+  int code;
+  SpgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// owning device allocation
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+    return *this;
+  }
+  ~DBuf();
+  void alloc(size_t n);
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct LevelRT {
+  LevelDev d;
+  DBuf nbr, flags, rsrc, psrc, base, bnd[8];
+  DBuf st64, st32, cell_row, cfb, cgi;
+  DBuf u32, b32, r32, u64, b64, r64;
+  int64_t n3 = 0;  // vector length nb*192
+  template <class T> T* u() const;
+  template <class T> T* b() const;
+  template <class T> T* r() const;
+  template <class T> const T* st() const;
+};
+
+class Solver {
+ public:
+  Solver(const spg_config& cfg, const ProblemDesc& pd);
+  ~Solver();
+
+  void set_forces(int64_t n, const int32_t* coord, const double* value);
+  void update_density(const double* density);
+  // returns SPG_OK or error code; fills info
+  int solve(const spg_solve_params& prm, spg_info* info);
+  int solve_adjoint(const spg_solve_params& prm, const double* g_dev_in3, double* lam_dev_in3, spg_info* info);
+  void pack_output(double* out4) const;  // host
+  void displacement_in3(double* dev) const;
+  double sensitivity(const double* u_in3_dev, const double* lam_in3_dev, double* dc_in_dev);
+
+  // kernel-level hooks
+  int levels() const { return lay_.nlev; }
+  const Layout& layout() const { return lay_; }
+  void run_kernel(int op, int level, int precision, int forward, const double* in, const double* b, double* out);
+  void level_stencils(int level, double* out);
+  double bench_apply(int precision, int reps);
+  const spg_config& config() const { return cfg_; }
+  double setup_seconds() const { return setup_s_; }
+  double device_bytes() const { return dev_bytes_; }
+
+ private:
+  template <class T> void vcycle(const T* b0, T* z);
+  template <class T> void smooth(int l, T* u, const T* b, bool forward);
+  template <class T> void level_residual(int l, const T* u, const T* b, T* r);
+  template <class T> void bottom(const T* b, T* u);
+  template <class T> int pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm);
+  template <class T> const T* E() const;
+  template <class T> T dxT() const { return (T)cfg_.dx; }
+  int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool with_ud);
+  void build_operators();
+  void upload_layout();
+  double read_scalar(const double* dev);
+  void sync();
+
+  spg_config cfg_;
+  Layout lay_;
+  std::vector<LevelRT> lv_;
+  double K0_[576];
+  hipStream_t st_ = nullptr;
+  rocblas_handle rb_ = nullptr;
+  // level-0 data
+  DBuf E64_, E32_, rho_, in_slot_, ud_idx_, ud_val_, f_idx_, f_val_;
+  int64_t n_ud_ = 0, n_f_ = 0;
+  // bottom
+  int64_t nvar_ = 0;
+  DBuf var_of_, Ainv_, xb_, yb_, info_;
+  // PCG / outer vectors (level 0)
+  DBuf x32_, rr32_, s32_, q32_, z32_, x64_, rr64_, s64_, q64_, z64_;
+  DBuf u_, f_, res_, tmp_;
+  DBuf scal_, part_, pinned_;
+  double* hscal_ = nullptr;
+  double setup_s_ = 0.0;
+  double dev_bytes_ = 0.0;
+  std::vector<std::vector<double>> posmat_cache_;
+  std::unordered_map<uint64_t, int32_t> gmap0_;
+  bool have_solution_ = false;
+};
+
+}  // namespace spg

@@ -1,0 +1,236 @@
+"""Python driver seat (reference: scripts/topo_opt.py, topo_opt/spgrid_topo_opt.cpp
+iterate() at :1289-1331): differentiable GPU FEM solves for torch.
+
+* :class:`Compliance` -- ``torch.autograd.Function`` whose forward is the GPU
+  MG-PCG solve and returns the reference-scaled compliance
+  c = sum_e u_e^T K0 u_e E(rho_e)  (= f.u / dx), and whose backward is the
+  hand-written sensitivity kernel dc/drho_e = -p rho_e^(p-1)(1-Emin) u_e^T K0 u_e
+  (objective/minimal_compliance.cpp:74-146).  Only u is saved.
+* :class:`Displacement` -- forward returns the displacement field (input block
+  order); backward runs the adjoint MG-PCG solve K lambda = g on the same
+  hierarchy and returns -dx E'(rho_e) lambda_e^T K0 u_e, so any objective of u
+  (e.g. a load-node displacement) is differentiable.
+* :func:`filter_sensitivity` / :func:`oc_update` -- the density-weighted cone
+  filter (spgrid_topo_opt.cpp:527-574) and the bisection optimality-criterion
+  update (optimizer/optimality_criterion.cpp:19-146) on the GPU with torch ops.
+
+Densities are tensors of shape (nblocks*64,) in the input block order of the
+Problem (index block*64 + i*16 + j*4 + k), float64 on cuda.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class FEM:
+    """A GPU solver bound to one block topology, with solve settings."""
+
+    def __init__(self, prob, tolerance=1e-6, precision=0, inner_tolerance=1e-3, max_passes=30, max_iterations=500,
+                 device=0):
+        self.prob = prob
+        self.solver = _lib.Solver(prob, device=device)
+        self.nin = prob.block_base.shape[0] * 64
+        self.kw = dict(tolerance=tolerance, precision=precision, inner_tolerance=inner_tolerance if precision == 0
+                       else 0.0, max_passes=max_passes if precision == 0 else 1, max_iterations=max_iterations)
+        self.last_info = None
+        self._rho_host = None
+
+    def set_density(self, rho: torch.Tensor):
+        # cells of the design domain never reach zero density: the caller feeds
+        # max(rho, DBL_MIN) so the solver keeps them (spgrid_topo_opt.cpp:1117-1122)
+        r = rho.detach().to(torch.float64).cpu().numpy()
+        inside = self.prob.density.reshape(-1) > 0
+        r = np.where(inside, np.maximum(r, np.finfo(np.float64).tiny), r)
+        if self._rho_host is None or not np.array_equal(r, self._rho_host):
+            self.solver.update_density(r.reshape(-1, 64))
+            self._rho_host = r.copy()
+
+    def solve(self) -> dict:
+        _, info = self.solver.solve(want_output=False, **self.kw)
+        self.last_info = info
+        return info
+
+    def displacement(self, device) -> torch.Tensor:
+        u = torch.empty(self.nin * 3, dtype=torch.float64, device=device)
+        torch.cuda.synchronize(device)
+        self.solver.displacement_device(u.data_ptr())
+        return u
+
+    def sensitivity(self, u: torch.Tensor, lam: torch.Tensor = None):
+        dc = torch.empty(self.nin, dtype=torch.float64, device=u.device)
+        torch.cuda.synchronize(u.device)
+        obj = self.solver.sensitivity_device(dc.data_ptr(), lam_ptr=None if lam is None else lam.data_ptr(),
+                                             u_ptr=u.data_ptr())
+        return obj, dc
+
+    def adjoint(self, g: torch.Tensor) -> torch.Tensor:
+        g = g.detach().to(torch.float64).contiguous()
+        lam = torch.empty_like(g)
+        torch.cuda.synchronize(g.device)
+        kw = dict(self.kw)
+        self.solver.solve_adjoint_device(g.data_ptr(), lam.data_ptr(), **kw)
+        return lam
+
+
+class Compliance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, fem: FEM):
+        fem.set_density(rho)
+        fem.solve()
+        u = fem.displacement(rho.device)
+        obj, _ = fem.sensitivity(u)
+        ctx.fem = fem
+        ctx.save_for_backward(u)
+        return torch.tensor(obj, dtype=torch.float64, device=rho.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        (u,) = ctx.saved_tensors
+        _, dc = ctx.fem.sensitivity(u)
+        return grad * dc, None
+
+
+class Displacement(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, fem: FEM):
+        fem.set_density(rho)
+        fem.solve()
+        u = fem.displacement(rho.device)
+        ctx.fem = fem
+        ctx.save_for_backward(u)
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_u):
+        (u,) = ctx.saved_tensors
+        lam = ctx.fem.adjoint(grad_u)
+        _, g = ctx.fem.sensitivity(u, lam)
+        return g * ctx.fem.prob.dx, None
+
+
+def compliance(rho, fem):
+    return Compliance.apply(rho, fem)
+
+
+def displacement(rho, fem):
+    return Displacement.apply(rho, fem)
+
+
+# --------------------------------------------------------------------------
+# density filter and OC update (next to the solve on the same blocks)
+# --------------------------------------------------------------------------
+class CellGrid:
+    """Input blocks as a cell grid with a 27-block neighbour table (torch)."""
+
+    def __init__(self, prob, device="cuda"):
+        bb = prob.block_base // 4
+        key = {tuple(b): i for i, b in enumerate(bb.tolist())}
+        nbr = np.full((len(bb), 27), -1, np.int64)
+        for i, b in enumerate(bb.tolist()):
+            for s in range(27):
+                o = (s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1)
+                nbr[i, s] = key.get((b[0] + o[0], b[1] + o[1], b[2] + o[2]), -1)
+        self.nb = len(bb)
+        self.nbr = torch.as_tensor(nbr, device=device)
+        self.inside = torch.as_tensor(prob.density.reshape(-1) > 0, device=device)
+
+    def halo(self, v, width=1):
+        """(nb*64,) cell values -> (nb, 4+2w, 4+2w, 4+2w) with neighbour halos (0 outside)."""
+        w = width
+        v = v.reshape(self.nb, 4, 4, 4)
+        padded = torch.cat([v.new_zeros(1, 4, 4, 4), v], 0)  # index 0 = absent block
+        n = 4 + 2 * w
+        out = v.new_zeros(self.nb, n, n, n)
+        for s in range(27):
+            o = (s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1)
+            src = padded[self.nbr[:, s] + 1]
+            sl_dst, sl_src = [], []
+            for d in range(3):
+                if o[d] < 0:
+                    sl_dst.append(slice(0, w))
+                    sl_src.append(slice(4 - w, 4))
+                elif o[d] > 0:
+                    sl_dst.append(slice(4 + w, 4 + 2 * w))
+                    sl_src.append(slice(0, w))
+                else:
+                    sl_dst.append(slice(w, 4 + w))
+                    sl_src.append(slice(0, 4))
+            out[:, sl_dst[0], sl_dst[1], sl_dst[2]] = src[:, sl_src[0], sl_src[1], sl_src[2]]
+        return out
+
+
+def filter_sensitivity(grid: CellGrid, dc, rho, filter_size=1.5):
+    """dc1(x) = sum_o w(o) dc(x+o) rho(x+o) / (sum_o w(o) [x+o inside] (rho(x)+1e-3)),
+    w(o) = max(0, r - |o|) over |o_i| <= floor(r) (spgrid_topo_opt.cpp:527-574)."""
+    h = int(np.floor(filter_size))
+    inside = grid.inside.to(torch.float64)
+    hv = grid.halo(dc * rho * inside, h)
+    hm = grid.halo(inside, h)
+    num = torch.zeros(grid.nb, 4, 4, 4, dtype=torch.float64, device=dc.device)
+    den = torch.zeros_like(num)
+    for a in range(-h, h + 1):
+        for b in range(-h, h + 1):
+            for c in range(-h, h + 1):
+                w = max(0.0, filter_size - float(np.sqrt(a * a + b * b + c * c)))
+                if w == 0.0:
+                    continue
+                sl = (slice(None), slice(h + a, h + a + 4), slice(h + b, h + b + 4), slice(h + c, h + c + 4))
+                num += w * hv[sl]
+                den += w * hm[sl]
+    out = num.reshape(-1) / (den.reshape(-1) * (rho + 1e-3))
+    return torch.where(grid.inside, out, torch.zeros_like(out))
+
+
+def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0):
+    """Bisection on the Lagrange multiplier (optimizer/optimality_criterion.cpp:19-146):
+    new = clamp(old*sqrt(max(0,-dc)/mid), old-step, old+step), clamped to [min_density, 1];
+    the volume sum over inside cells must equal volume_fraction * #inside."""
+    target = volume_fraction * inside.sum().item()
+    neg = torch.clamp(-dc, min=0.0)
+
+    def trial(mid):
+        new = torch.clamp(rho * torch.sqrt(neg / mid), rho - step_limit, rho + step_limit)
+        new = torch.clamp(new, min_density, 1.0)
+        return torch.where(inside, new, rho)
+
+    lower, upper = 1e-20, 1e20
+    while lower * (1 + 1e-7) < upper:
+        mid = 0.5 * (lower + upper)
+        if trial(mid)[inside].sum().item() > target:
+            lower = mid
+        else:
+            upper = mid
+    new = trial(mid)
+    return new, (new - rho).abs().max().item()
+
+
+class TopoOpt:
+    """Minimal SIMP/OC loop on the GPU (reference iterate(): solve, sensitivity,
+    filter, OC update).  The block topology stays fixed."""
+
+    def __init__(self, prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, **fem_kw):
+        self.prob = prob
+        self.fem = FEM(prob, **fem_kw)
+        self.grid = CellGrid(prob)
+        self.volume_fraction = volume_fraction
+        self.step_limit = step_limit
+        self.filter_size = filter_size
+        inside = self.grid.inside
+        self.rho = torch.where(inside, torch.full_like(inside, volume_fraction, dtype=torch.float64),
+                               torch.zeros_like(inside, dtype=torch.float64))
+        self.history = []
+
+    def iterate(self):
+        rho = self.rho.clone().requires_grad_(True)
+        c = compliance(rho, self.fem)
+        c.backward()
+        dc = filter_sensitivity(self.grid, rho.grad.detach(), self.rho, self.filter_size)
+        new, change = oc_update(self.rho, dc, self.grid.inside, self.volume_fraction, self.step_limit)
+        self.rho = new
+        rec = {"compliance": c.item(), "change": change, "iterations": self.fem.last_info["iterations"],
+               "solve_s": self.fem.last_info["solve_seconds"]}
+        self.history.append(rec)
+        return rec

@@ -86,41 +86,51 @@ class Problem:
         """Sorted unique (M,3) coords of nodes touching an active cell."""
         cc, _ = self.cell_coords()
         offs = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.int32)
-        nodes = (cc[:, None, :] + offs[None]).reshape(-1, 3)
-        return np.unique(nodes, axis=0)
+        return _unpack(np.unique(_pack((cc[:, None, :] + offs[None]).reshape(-1, 3)))).astype(np.int32)
 
     def with_levels(self, mg_levels):
         return dataclasses.replace(self, mg_levels=mg_levels)
+
+
+_KB = np.int64(1 << 20)
+
+
+def _pack(c):
+    c = np.asarray(c, np.int64)
+    return (c[..., 0] * _KB + c[..., 1]) * _KB + c[..., 2]
+
+
+def _unpack(k):
+    k = np.asarray(k, np.int64)
+    return np.stack([k // (_KB * _KB), (k // _KB) % _KB, k % _KB], -1)
 
 
 def _blocks_from_cells(cells, dens):
     """Group cells (SPGrid coords) into 4^3 blocks and add node-only blocks."""
     cells = np.asarray(cells, np.int64)
     bkey = cells >> 2
-    offs = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.int64)
-    node_blocks = ((cells[:, None, :] + offs[None]) >> 2).reshape(-1, 3)
-    allb = np.unique(np.concatenate([bkey, node_blocks]), axis=0)
-    # lexicographic order of blocks (x slowest)
-    nb = allb.shape[0]
-    M = np.int64(1 << 20)
-    keys = (allb[:, 0] * M + allb[:, 1]) * M + allb[:, 2]
-    order = np.argsort(keys)
-    allb = allb[order]
-    keys = keys[order]
-    ck = (bkey[:, 0] * M + bkey[:, 1]) * M + bkey[:, 2]
+    ck = _pack(bkey)
+    keys = [np.unique(ck)]
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                if a or b or c:
+                    keys.append(np.unique(_pack((cells + np.array([a, b, c])) >> 2)))
+    keys = np.unique(np.concatenate(keys))  # sorted: lexicographic block order (x slowest)
     bid = np.searchsorted(keys, ck)
     loc = cells & 3
     li = loc[:, 0] * 16 + loc[:, 1] * 4 + loc[:, 2]
-    density = np.zeros((nb, 64), np.float64)
+    density = np.zeros((keys.shape[0], 64), np.float64)
     density[bid, li] = dens
-    return (allb * 4).astype(np.int32), density
+    return (_unpack(keys) * 4).astype(np.int32), density
 
 
 def _clamp_cells(cells):
     """Dirichlet nodes (all 3 axes) of whole clamped cells, deduplicated."""
     cells = np.asarray(cells, np.int32).reshape(-1, 3)
     offs = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.int32)
-    nodes = np.unique((cells[:, None, :] + offs[None]).reshape(-1, 3), axis=0)
+    nodes = _unpack(np.unique(_pack((cells[:, None, :] + offs[None]).reshape(-1, 3)))) if len(cells) else \
+        np.zeros((0, 3), np.int64)
     coord = np.repeat(nodes, 3, axis=0)
     axis = np.tile(np.arange(3, dtype=np.int32), nodes.shape[0])
     return coord.astype(np.int32), axis, np.zeros(coord.shape[0])
@@ -204,10 +214,12 @@ def sphere_shell(n=256, r_in=0.4, r_out=0.5, seed=0, lo=0.05, hi=1.0, filter_pas
     dist = np.linalg.norm(cen - south, axis=1)
     clamp = cells[dist <= clamp_radius + dx]
     dc, da, dv = _clamp_cells(clamp)
-    offs = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.int64)
-    nodes = np.unique((ijk[:, None, :] + offs[None]).reshape(-1, 3), axis=0)
-    npos = nodes * dx - 0.5
+    # active node nearest the north pole: among nodes of the cells closest to it
     north = np.array([0.0, 0.5, 0.0])
+    near = ijk[np.linalg.norm(cen - north, axis=1) <= 2 * dx + np.linalg.norm(cen - north, axis=1).min()]
+    offs = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.int64)
+    nodes = _unpack(np.unique(_pack((near[:, None, :] + offs[None]).reshape(-1, 3))))
+    npos = nodes * dx - 0.5
     lnode = nodes[np.argmin(np.linalg.norm(npos - north, axis=1))] + OFFSET
     fc = lnode.reshape(1, 3).astype(np.int32)
     fv = np.array([[0.0, -1.0, 0.0]])

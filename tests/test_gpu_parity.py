@@ -1,0 +1,289 @@
+"""GPU parity: every HIP kernel and the full solves against the C oracle.
+
+Vectors are exchanged by node coordinates: the product reports its level-node
+order (spg_level_nodes), the oracle its allocated nodes; values are mapped
+through the coordinates.  FP64 kernels must agree to ~1e-12 relative, FP32
+kernels to ~1e-5; solves are compared through displacement, residual and
+compliance errors (tolerances stated per test).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spgrid_topo_opt_amd import _lib
+from spgrid_topo_opt_amd import fem_interface as FI
+from spgrid_topo_opt_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _key(xyz):
+    xyz = np.asarray(xyz, np.int64) + 4096
+    return (xyz[:, 0] << 42) | (xyz[:, 1] << 21) | xyz[:, 2]
+
+
+class Pair:
+    """A problem built on both sides plus coordinate maps per level."""
+
+    def __init__(self, prob):
+        print(f"[pair] {prob.name} levels={prob.mg_levels}", flush=True)
+        self.p = prob
+        self.o = O.Oracle(prob)
+        self.g = _lib.Solver(prob)
+        self.map = []
+        for lv in range(self.o.levels):
+            gx, gf = self.g.level_nodes(lv)
+            ox = self.o.xyz(lv)
+            ok = _key(ox)
+            order = np.argsort(ok)
+            pos = np.searchsorted(ok[order], _key(gx))
+            assert (pos < len(ok)).all()
+            idx = order[pos]
+            assert (ok[idx] == _key(gx)).all(), "product node missing from the oracle's allocated set"
+            self.map.append((idx, gf))
+
+    def to_o(self, lv, v):
+        out = np.zeros((self.o.nnodes(lv), 3))
+        out[self.map[lv][0]] = v
+        return out
+
+    def from_o(self, lv, v):
+        return np.asarray(v)[self.map[lv][0]]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def blocky():
+    return Pair(W.random_blocky(seed=11, n=(16, 12, 12), mg_levels=4))
+
+
+@pytest.fixture(scope="module")
+def box3():
+    return Pair(W.box(16, 8, 8, seed=1, mg_levels=3))
+
+
+def test_flags_match(blocky):
+    P = blocky
+    for lv in range(P.o.levels):
+        idx, gf = P.map[lv]
+        of = P.o.flags(lv)[idx]
+        want = ((of & 0x10) != 0) * 1 | ((of >> 5) & 7) << 1 | ((of & 0x1000) != 0) * 16
+        np.testing.assert_array_equal(gf, want.astype(np.uint8))
+        # every active oracle node is present on the GPU
+        assert ((P.o.flags(lv) & 0x10) != 0).sum() == ((gf & 1) != 0).sum()
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 2e-6)])
+def test_apply_and_residual(blocky, prec, tol):
+    P = blocky
+    rng = np.random.default_rng(0)
+    n = len(P.map[0][0])
+    u = rng.standard_normal((n, 3))
+    b = rng.standard_normal((n, 3))
+    got = P.g.kernel(_lib.OP_APPLY, 0, u, precision=prec)
+    want = P.from_o(0, P.o.apply(P.to_o(0, u)))
+    assert rel(got, want) < tol
+    got = P.g.kernel(_lib.OP_RESIDUAL, 0, u, b, precision=prec)
+    want_full = P.to_o(0, b) - P.o.apply(P.to_o(0, u))
+    dm = (P.o.flags(0)[:, None] >> (5 + np.arange(3))) & 1
+    want_full[dm.astype(bool)] = 0
+    assert rel(got, P.from_o(0, want_full)) < tol * 10
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
+@pytest.mark.parametrize("forward,boundary", [(True, False), (False, False), (True, True), (False, True)])
+def test_gs_sweep_level0(blocky, prec, tol, forward, boundary):
+    P = blocky
+    rng = np.random.default_rng(5)
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    u = rng.standard_normal((n, 3)) * 1e-6
+    u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    b = rng.standard_normal((n, 3))
+    op = _lib.OP_GS_BOUNDARY if boundary else _lib.OP_GS_SWEEP
+    got = P.g.kernel(op, 0, u, b, precision=prec, forward=forward)
+    want = P.from_o(0, P.o.gs_sweep(0, P.to_o(0, u), P.to_o(0, b), forward, boundary))
+    assert rel(got, want) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 1e-6)])
+def test_restriction_and_prolongation(blocky, prec, tol):
+    P = blocky
+    rng = np.random.default_rng(7)
+    for lv in range(1, P.o.levels):
+        nf, nc = len(P.map[lv - 1][0]), len(P.map[lv][0])
+        r = rng.standard_normal((nf, 3))
+        got = P.g.kernel(_lib.OP_RESTRICT, lv, r, precision=prec)
+        want = P.from_o(lv, P.o.restrict(lv, P.to_o(lv - 1, r)))
+        assert rel(got, want) < tol, lv
+        uc = rng.standard_normal((nc, 3))
+        uf = rng.standard_normal((nf, 3))
+        got = P.g.kernel(_lib.OP_PROLONGATE, lv - 1, uc, uf, precision=prec)
+        want = P.o.prolongate(lv - 1, P.to_o(lv - 1, uf), P.to_o(lv, uc))
+        want[(P.o.flags(lv - 1) & 0x10) == 0] = 0  # null-space projection fused in the kernel
+        assert rel(got, P.from_o(lv - 1, want)) < tol, lv
+
+
+def test_galerkin_stencils_fp64(blocky):
+    P = blocky
+    for lv in range(2, P.o.levels):
+        got = P.g.stencils(lv)
+        want = P.o.stencils(lv)[P.map[lv][0]]
+        act = (P.map[lv][1] & 1) != 0
+        assert act.sum() > 0
+        assert rel(got[act], want[act]) < 1e-12, lv
+        assert np.abs(got[~act]).max() == 0
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 2e-5)])
+def test_stencil_gs_and_residual(blocky, prec, tol):
+    P = blocky
+    rng = np.random.default_rng(9)
+    for lv in range(2, P.o.levels):
+        n = len(P.map[lv][0])
+        gf = P.map[lv][1]
+        u = rng.standard_normal((n, 3)) * 1e-3
+        u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+        u[(gf & 1) == 0] = 0
+        b = rng.standard_normal((n, 3))
+        got = P.g.kernel(_lib.OP_STENCIL_RES, lv, u, b, precision=prec)
+        want = P.from_o(lv, P.o.stencil_residual(lv, P.to_o(lv, u), P.to_o(lv, b)))
+        assert rel(got, want) < tol
+        for fwd in (True, False):
+            got = P.g.kernel(_lib.OP_GS_SWEEP, lv, u, b, precision=prec, forward=fwd)
+            want = P.from_o(lv, P.o.gs_sweep(lv, P.to_o(lv, u), P.to_o(lv, b), fwd, False))
+            assert rel(got, want) < tol * 10
+
+
+def test_bottom_solve_is_exact(blocky):
+    P = blocky
+    B = P.o.levels - 1
+    rng = np.random.default_rng(3)
+    n = len(P.map[B][0])
+    gf = P.map[B][1]
+    b = rng.standard_normal((n, 3))
+    b[(gf & 1) == 0] = 0
+    b[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    x = P.g.kernel(_lib.OP_BOTTOM, B, None, b, precision=1)
+    res = P.o.stencil_residual(B, P.to_o(B, x), P.to_o(B, b))
+    assert np.abs(res).max() < 1e-9 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-10), (0, 1e-4)])
+def test_vcycle(blocky, prec, tol):
+    P = blocky
+    rng = np.random.default_rng(13)
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    b = rng.standard_normal((n, 3))
+    b[(gf & 1) == 0] = 0
+    b[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    got = P.g.kernel(_lib.OP_VCYCLE, 0, None, b, precision=prec)
+    want = P.from_o(0, P.o.vcycle(P.to_o(0, b), O.MODE_FP64))
+    assert rel(got, want) < tol
+
+
+def _solution_error(P, out4, u_o):
+    """relative L2 error of the displacement at the input block nodes"""
+    want = P.o.pack(u_o)[..., :3]
+    got = out4[..., :3]
+    return np.linalg.norm(got - want) / np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("levels", [0, 1, 3, 4])
+def test_fp64_solve_matches_oracle(levels):
+    P = Pair(W.random_blocky(seed=21, n=(16, 12, 12), mg_levels=levels))
+    tol = 1e-9 if levels == 0 else 1e-10
+    u_o, r_o, io = P.o.solve(tolerance=tol, max_iterations=5000)
+    assert io["rc"] == 0
+    out, ig = P.g.solve(tolerance=tol, max_iterations=5000, precision=1)
+    print("oracle", io["iterations"], "gpu", ig["iterations"], flush=True)
+    assert ig["converged"] == 1
+    assert abs(ig["iterations"] - io["iterations"]) <= max(2, io["iterations"] // 20)
+    assert _solution_error(P, out, u_o) < 1e-7
+    assert ig["r_linf"] <= tol * ig["r0_linf"]
+
+
+def test_mixed_solve_matches_fp64_oracle(box3):
+    P = box3
+    u_o, r_o, io = P.o.solve(tolerance=1e-12, max_iterations=500)
+    out, ig = P.g.solve_mixed(tolerance=1e-6)
+    assert ig["converged"] == 1 and ig["passes"] >= 1
+    assert ig["r_linf"] <= 1e-6 * ig["r0_linf"]
+    err = _solution_error(P, out, u_o)
+    assert err < 1e-5, err
+    # compliance f.u / dx against the oracle's (fem objective scale: f.u/dx = sum_e s_e E_e)
+    c_o = P.o.sensitivity(u_o)[0]
+    idx = {tuple(c): i for i, c in enumerate(P.o.xyz(0))}
+    fu_o = sum(np.dot(v, u_o[idx[tuple(c)]]) for c, v in zip(P.p.force_coord, P.p.force_value))
+    nodes = (P.p.block_base[:, None, :] + np.stack(np.meshgrid(range(4), range(4), range(4), indexing="ij"),
+                                                   -1).reshape(64, 3)[None]).reshape(-1, 3)
+    nidx = {tuple(c): i for i, c in enumerate(nodes)}
+    fu_g = sum(np.dot(v, out.reshape(-1, 4)[nidx[tuple(c)], :3]) for c, v in zip(P.p.force_coord, P.p.force_value))
+    assert abs(fu_g - fu_o) / abs(fu_o) < 1e-6
+    assert abs(fu_o / P.p.dx - c_o) / c_o < 1e-9
+
+
+def test_cfg1_reference_pass_and_mixed():
+    P = Pair(W.config("cfg1"))
+    u_o, _, io = P.o.solve(tolerance=1e-10, max_iterations=500)
+    # the reference's single pass at krylov tolerance 1e-4 (FP32 inner)
+    out1, i1 = P.g.solve(tolerance=1e-4, max_iterations=500, precision=0)
+    assert i1["converged"] == 1 and i1["passes"] == 1
+    assert _solution_error(P, out1, u_o) < 1e-3
+    out2, i2 = P.g.solve_mixed(tolerance=1e-6)
+    assert i2["r_linf"] <= 1e-6 * i2["r0_linf"]
+    assert _solution_error(P, out2, u_o) < 1e-5
+    print("cfg1 iterations: oracle(1e-10)", io["iterations"], "ref-pass", i1["iterations"], "mixed", i2["iterations"],
+          "passes", i2["passes"], flush=True)
+
+
+def test_fem_solve_end_to_end(box3):
+    P = box3
+    u_o, r_o, io = P.o.solve(tolerance=1e-10, max_iterations=500)
+    inp = FI.FEMInputs.from_problem(P.p, tolerance=1e-10, max_iterations=500,
+                                    extra_ints={"defect_correction_iter": 20}, extra_doubles={"inner_tolerance": 1e-3})
+    outp = FI.FEMInterface(inp).run()
+    assert outp.success, outp.info
+    assert (outp.displacement_base == P.p.block_base).all()
+    want = P.o.pack(u_o)
+    err = np.linalg.norm(outp.displacements[..., :3] - want[..., :3]) / np.linalg.norm(want[..., :3])
+    assert err < 1e-8, err
+    assert outp.residuals and outp.residuals[-1] <= 1e-10
+
+
+def test_sensitivity_matches_oracle(box3):
+    torch = pytest.importorskip("torch")
+    P = box3
+    out, ig = P.g.solve(tolerance=1e-10, max_iterations=500, precision=1)
+    nin = P.p.block_base.shape[0] * 64
+    dc = torch.zeros(nin, dtype=torch.float64, device="cuda")
+    obj = P.g.sensitivity_device(dc.data_ptr())
+    torch.cuda.synchronize()
+    u_o, _, _ = P.o.solve(tolerance=1e-12, max_iterations=500)
+    obj_o, dc_o, _ = P.o.sensitivity(u_o)
+    assert abs(obj - obj_o) / obj_o < 1e-8
+    # map oracle per-node cells to input-block order
+    nodes = (P.p.block_base[:, None, :] + np.stack(np.meshgrid(range(4), range(4), range(4), indexing="ij"),
+                                                   -1).reshape(64, 3)[None]).reshape(-1, 3)
+    oi = {tuple(c): i for i, c in enumerate(P.o.xyz(0))}
+    want = np.array([dc_o[oi[tuple(c)]] if tuple(c) in oi else 0.0 for c in nodes])
+    got = dc.cpu().numpy()
+    assert rel(got, want) < 1e-7
+
+
+def test_errors_are_reported():
+    # floating body (no Dirichlet cells): CG detects a singular system or does not converge
+    p = W.box(8, 4, 4)
+    q = W.Problem(**{**p.__dict__, "dir_coord": p.dir_coord[:0], "dir_axis": p.dir_axis[:0],
+                     "dir_value": p.dir_value[:0], "mg_levels": 0})
+    s = _lib.Solver(q)
+    _, info = s.solve(tolerance=1e-12, max_iterations=50, precision=1, raise_on_error=False)
+    assert info["rc"] in (_lib.SPG_E_SINGULAR, _lib.SPG_E_NOT_CONVERGED)
+    with pytest.raises(_lib.SpgError):
+        _lib.Solver(p.with_levels(2))
