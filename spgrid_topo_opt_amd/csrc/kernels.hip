@@ -6,8 +6,10 @@
 // SPGrid pages do in the reference.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdio>
+#include <cstdlib>
 
 #include "kernels.hpp"
 
@@ -59,6 +61,18 @@ __device__ __forceinline__ int ring_block(const Ring& R, int a, int b, int c) {
   return s == 13 ? R.blk : R.t[s];
 }
 __device__ inline int loc3(int a, int b, int c) { return ((a & 3) << 4) | ((b & 3) << 2) | (c & 3); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this renumbering
+// gives each XCD one contiguous range of logical workgroups (and thus of
+// Morton-ordered blocks), so neighbouring blocks share an L2.
+__device__ __forceinline__ int xcd_chunk(int h, int n) {
+  const int q = n >> 3, r = n & 7, x = h & 7, i = h >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+__device__ __forceinline__ int ring_side(int x) { return x < 0 ? 0 : (x > 3 ? 2 : 1); }
+__device__ __forceinline__ int ring_spoke(int x, int y, int z) {
+  return ring_side(x) * 9 + ring_side(y) * 3 + ring_side(z);
+}
 
 // ---------------------------------------------------------------- level 0
 // Row of the matrix-free operator at one node (SPGrid_Linear_Elasticity.h:
@@ -142,6 +156,79 @@ __global__ void __launch_bounds__(TPB) k_apply(int nb, const int* __restrict__ n
   }
 }
 
+// LDS-staged apply: a wavefront owns one block; it stages the 6^3 nodes and
+// the 5^3 cells around the block, then lane l evaluates the row of node l
+// from LDS.  All lanes visit the cells I = 0..7 in step, so K0[I][J] are
+// wave-uniform constants.
+static constexpr int AP_WAVES = 4;
+template <class T, int MODE>
+__global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* __restrict__ nbr,
+                                                             const uint8_t* __restrict__ flags,
+                                                             const T* __restrict__ E, T dx, const T* __restrict__ in,
+                                                             const T* __restrict__ b, T* __restrict__ out) {
+  __shared__ T hu[AP_WAVES][3][216];
+  __shared__ T he[AP_WAVES][128];
+  __shared__ int ht[AP_WAVES][28];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_chunk(blockIdx.x, gridDim.x) * AP_WAVES + w;
+  const bool live = blk < nb;
+  if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  __syncthreads();
+  for (int h = lane; h < 216; h += 64) {
+    const int x = h / 36 - 1, y = (h / 6) % 6 - 1, z = h % 6 - 1;
+    const int bb = ht[w][ring_spoke(x, y, z)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? in[vidx(bb, c, q)] : T(0);
+  }
+  for (int h = lane; h < 125; h += 64) {
+    const int x = h / 25 - 1, y = (h / 5) % 5 - 1, z = h % 5 - 1;
+    const int bb = ht[w][ring_spoke(x, y, z)];
+    he[w][h] = bb >= 0 ? E[bb * 64 + loc3(x, y, z)] : T(0);
+  }
+  __syncthreads();
+  const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+  T uu[27][3];
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    const int h = (li + s / 9) * 36 + (lj + (s / 3) % 3) * 6 + (lk + s % 3);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uu[s][c] = hu[w][c][h];
+  }
+  T f0 = 0, f1 = 0, f2 = 0;
+#pragma unroll
+  for (int I = 0; I < 8; ++I) {
+    const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
+    const T e = he[w][(li - ix + 1) * 25 + (lj - iy + 1) * 5 + (lk - iz + 1)];
+    T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int J = 0; J < 8; ++J) {
+      const int jx = (J >> 2) & 1, jy = (J >> 1) & 1, jz = J & 1;
+      const int sp = (jx - ix + 1) * 9 + (jy - iy + 1) * 3 + (jz - iz + 1);
+      const int kb = (I * 8 + J) * 9;
+      t0 += KT<T>::k(kb + 0) * uu[sp][0] + KT<T>::k(kb + 1) * uu[sp][1] + KT<T>::k(kb + 2) * uu[sp][2];
+      t1 += KT<T>::k(kb + 3) * uu[sp][0] + KT<T>::k(kb + 4) * uu[sp][1] + KT<T>::k(kb + 5) * uu[sp][2];
+      t2 += KT<T>::k(kb + 6) * uu[sp][0] + KT<T>::k(kb + 7) * uu[sp][1] + KT<T>::k(kb + 8) * uu[sp][2];
+    }
+    f0 += e * t0;
+    f1 += e * t1;
+    f2 += e * t2;
+  }
+  if (!live) return;
+  const T f[3] = {dx * f0, dx * f1, dx * f2};
+  if (MODE == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[vidx(blk, c, lane)] = f[c];
+  } else {
+    const uint8_t fl = flags[blk * 64 + lane];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = MODE == 2 ? f[c] : b[vidx(blk, c, lane)] - f[c];
+      out[vidx(blk, c, lane)] = (fl & (2 << c)) ? T(0) : v;
+    }
+  }
+}
+
 // symmetric 3x3 inverse by minors with Dirichlet rows/cols handled as in
 // SPGrid_Gauss_Seidel.h:1093-1144 / Multigrid_Hierarchy.h:64-101.
 template <class T>
@@ -159,6 +246,27 @@ __device__ inline void dirichlet_inverse(T d[6], uint8_t fl) {
   if (fl & 8) { d[2] = 0; d[4] = 0; d[5] = 0; }
 }
 
+// u <- u + D^-1 (b - f) on the active, non-Dirichlet components of one node;
+// Dirichlet components are forced to 0 (also on inactive nodes)
+template <class T>
+__device__ inline void gs_node_update(uint8_t fl, T d[6], T f0, T f1, T f2, int blk, int l, T u0, T u1, T u2,
+                                      const T* __restrict__ b, T* __restrict__ u) {
+  if (fl & 1) {
+    dirichlet_inverse(d, fl);
+    const T r0 = b[vidx(blk, 0, l)] - f0, r1 = b[vidx(blk, 1, l)] - f1, r2 = b[vidx(blk, 2, l)] - f2;
+    const T n0 = u0 + d[0] * r0 + d[3] * r1 + d[4] * r2;
+    const T n1 = u1 + d[3] * r0 + d[1] * r1 + d[5] * r2;
+    const T n2 = u2 + d[4] * r0 + d[5] * r1 + d[2] * r2;
+    u[vidx(blk, 0, l)] = (fl & 2) ? T(0) : n0;
+    u[vidx(blk, 1, l)] = (fl & 4) ? T(0) : n1;
+    u[vidx(blk, 2, l)] = (fl & 8) ? T(0) : n2;
+  } else if (fl & 0x0E) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (fl & (2 << c)) u[vidx(blk, c, l)] = T(0);
+  }
+}
+
 // the k-th node (0..7) of colour c in a 4x4x4 block
 __device__ inline int colour_node(int colour, int k) {
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
@@ -166,126 +274,554 @@ __device__ inline int colour_node(int colour, int k) {
   return (i << 4) | (j << 2) | kk;
 }
 
+// ------------------------------------------------------- lane-octet helpers
+// Gauss-Seidel kernels give every updated node an aligned octet of 8 lanes
+// (lane bits 0..2).  octet_sum adds a value over the octet with three DPP
+// steps; all 8 lanes receive the same bits.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float x) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double x) {
+  const long long v = __double_as_longlong(x);
+  const int lo = dpp_i<CTRL>((int)(v & 0xffffffffll)), hi = dpp_i<CTRL>((int)(v >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
 template <class T>
-__global__ void __launch_bounds__(TPB) k_gs0(int nb, int colour, const int* __restrict__ nbr,
-                                             const uint8_t* __restrict__ flags, const T* __restrict__ E, T dx, T* u,
-                                             const T* __restrict__ b, const int* __restrict__ list, int nlist) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  int blk, l;
-  if (list) {
-    if (gid >= nlist) return;
-    const int s = list[gid];
-    blk = s >> 6;
-    l = s & 63;
-  } else {
-    if (gid >= nb * 8) return;
-    blk = gid >> 3;
-    l = colour_node(colour, gid & 7);
-  }
-  const uint8_t fl = flags[blk * 64 + l];
-  if (fl & 1) {
-    T f[3], d[6];
-    row0<T, true>(nbr, E, u, blk, l, f, d);
-    T r0 = b[vidx(blk, 0, l)] - dx * f[0];
-    T r1 = b[vidx(blk, 1, l)] - dx * f[1];
-    T r2 = b[vidx(blk, 2, l)] - dx * f[2];
+__device__ __forceinline__ T octet_sum(T x) {
+  x += dpp_t<0xB1>(x);   // quad_perm(1,0,3,2): partner lane ^ 1
+  x += dpp_t<0x4E>(x);   // quad_perm(2,3,0,1): partner lane ^ 2
+  x += dpp_t<0x141>(x);  // row_half_mirror: partner lane 7 - i of the octet
+  return x;
+}
+
+
+// ------------------------------------------------- level-0 Gauss-Seidel
+// Row of corner I of one cell, by reflection: the Q1 cube is symmetric under
+// the mirror x_d -> 1 - x_d, so K0[I][J]_{vw} = s_v(I) s_w(I) K0[0][I^J]_{vw}
+// with s_d(I) = -1 if bit d of I is set.  Only the 72 entries K0[0][*] are
+// needed and they are wave-uniform (scalar registers).  `U(J)` returns the 3
+// components of the cell's node J.  Result: the 3 rows times e (no dx).
+template <class T, class LoadU>
+__device__ __forceinline__ void corner_row(int I, T e, const LoadU& U, T& f0, T& f1, T& f2) {
+  const T sx = (I & 4) ? T(-1) : T(1), sy = (I & 2) ? T(-1) : T(1), sz = (I & 1) ? T(-1) : T(1);
+  T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-    for (int e = 0; e < 6; ++e) d[e] *= dx;
-    dirichlet_inverse(d, fl);
-    const T u0 = u[vidx(blk, 0, l)] + d[0] * r0 + d[3] * r1 + d[4] * r2;
-    const T u1 = u[vidx(blk, 1, l)] + d[3] * r0 + d[1] * r1 + d[5] * r2;
-    const T u2 = u[vidx(blk, 2, l)] + d[4] * r0 + d[5] * r1 + d[2] * r2;
-    u[vidx(blk, 0, l)] = (fl & 2) ? T(0) : u0;
-    u[vidx(blk, 1, l)] = (fl & 4) ? T(0) : u1;
-    u[vidx(blk, 2, l)] = (fl & 8) ? T(0) : u2;
-  } else if (fl & 0x0E) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      if (fl & (2 << c)) u[vidx(blk, c, l)] = T(0);
+  for (int Jp = 0; Jp < 8; ++Jp) {
+    T v0, v1, v2;
+    U(I ^ Jp, v0, v1, v2);
+    v0 *= sx;
+    v1 *= sy;
+    v2 *= sz;
+    const int kb = Jp * 9;
+    a0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
+    a1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
+    a2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
   }
+  f0 = sx * e * a0;
+  f1 = sy * e * a1;
+  f2 = sz * e * a2;
+}
+
+// Octet reduction of the row and of the 3x3 self block
+// D = sum_I e_I K0[I][I] (SPGrid_Gauss_Seidel.h:765-875), scaled by dx.
+template <class T>
+__device__ __forceinline__ void octet_row_and_diag(int I, T e, T dx, T& f0, T& f1, T& f2, T d[6]) {
+  const T sx = (I & 4) ? T(-1) : T(1), sy = (I & 2) ? T(-1) : T(1), sz = (I & 1) ? T(-1) : T(1);
+  f0 = dx * octet_sum(f0);
+  f1 = dx * octet_sum(f1);
+  f2 = dx * octet_sum(f2);
+  const T S = dx * octet_sum(e), Sxy = dx * octet_sum(sx * sy * e), Sxz = dx * octet_sum(sx * sz * e),
+          Syz = dx * octet_sum(sy * sz * e);
+  d[0] = KT<T>::k(0) * S;
+  d[1] = KT<T>::k(4) * S;
+  d[2] = KT<T>::k(8) * S;
+  d[3] = KT<T>::k(1) * Sxy;
+  d[4] = KT<T>::k(2) * Sxz;
+  d[5] = KT<T>::k(5) * Syz;
+}
+
+static constexpr int GS_WAVES = 4;  // blocks (wavefronts) per workgroup in the colour sweeps
+
+// One colour of the level-0 multicolour sweep over all blocks
+// (SPGrid_Gauss_Seidel.h:765-875).  A wavefront owns one block: it stages
+// the 5^3 nodes around the block's 8 colour nodes in LDS, then lane
+// (k, I) = (lane>>3, lane&7) adds the row of colour node k in the cell of
+// which the node is corner I; the octet sums, and lane I=0 updates the node.
+// A halo never holds nodes of the sweep's colour outside its own block, so
+// waves do not race.
+template <class T>
+__global__ void __launch_bounds__(64 * GS_WAVES) k_gs0_colour(int nb, int colour, const int* __restrict__ nbr,
+                                                              const uint8_t* __restrict__ flags,
+                                                              const T* __restrict__ E, T dx, T* __restrict__ u,
+                                                              const T* __restrict__ b) {
+  __shared__ T hu[GS_WAVES][3][128];
+  __shared__ int ht[GS_WAVES][28];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_chunk(blockIdx.x, gridDim.x) * GS_WAVES + w;
+  const bool live = blk < nb;
+  const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
+  if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  __syncthreads();
+  for (int h = lane; h < 125; h += 64) {
+    const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
+    const int bb = ht[w][ring_spoke(x, y, z)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+  }
+  __syncthreads();
+  const int k = lane >> 3, I = lane & 7;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;  // halo coords
+  const int cx = nx - ((I >> 2) & 1), cy = ny - ((I >> 1) & 1), cz = nz - (I & 1);           // cell corner
+  const int ex = px - 1 + cx, ey = py - 1 + cy, ez = pz - 1 + cz;                             // block coords
+  const int eb = ht[w][ring_spoke(ex, ey, ez)];
+  const T e = eb >= 0 ? E[eb * 64 + loc3(ex, ey, ez)] : T(0);
+  const T* H = &hu[w][0][0];
+  auto U = [&](int J, T& v0, T& v1, T& v2) {
+    const int h = ((cx + ((J >> 2) & 1)) * 5 + (cy + ((J >> 1) & 1))) * 5 + (cz + (J & 1));
+    v0 = H[h];
+    v1 = H[128 + h];
+    v2 = H[256 + h];
+  };
+  T f0, f1, f2, d[6];
+  corner_row<T>(I, e, U, f0, f1, f2);
+  octet_row_and_diag<T>(I, e, dx, f0, f1, f2, d);
+  if (I == 0 && live) {
+    const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+    const int hn = (nx * 5 + ny) * 5 + nz;
+    gs_node_update<T>(flags[blk * 64 + l], d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], b, u);
+  }
+}
+
+// Level-0 update of list node `s` (blk*64+l) by the 8 lanes of an octet,
+// reading straight from global memory (boundary lists).
+template <class T>
+__device__ __forceinline__ void gs0_octet_global(bool live, int s, int I, const int* __restrict__ nbr,
+                                                 const uint8_t* __restrict__ flags, const T* __restrict__ E, T dx,
+                                                 T* __restrict__ u, const T* __restrict__ b) {
+  const int blk = live ? (s >> 6) : 0, l = s & 63;
+  const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
+  const int cx = li - ((I >> 2) & 1), cy = lj - ((I >> 1) & 1), cz = lk - (I & 1);  // block coords of the cell
+  const int* t = nbr + blk * 27;
+  auto ring = [&](int x, int y, int z) {
+    const int sp = ring_spoke(x, y, z);
+    return !live ? -1 : (sp == 13 ? blk : t[sp]);
+  };
+  const int eb = ring(cx, cy, cz);
+  const T e = eb >= 0 ? E[eb * 64 + loc3(cx, cy, cz)] : T(0);
+  auto U = [&](int J, T& v0, T& v1, T& v2) {
+    const int x = cx + ((J >> 2) & 1), y = cy + ((J >> 1) & 1), z = cz + (J & 1);
+    const int bb = ring(x, y, z);
+    const int q = loc3(x, y, z);
+    v0 = bb >= 0 ? u[vidx(bb, 0, q)] : T(0);
+    v1 = bb >= 0 ? u[vidx(bb, 1, q)] : T(0);
+    v2 = bb >= 0 ? u[vidx(bb, 2, q)] : T(0);
+  };
+  T f0, f1, f2, d[6];
+  corner_row<T>(I, e, U, f0, f1, f2);
+  octet_row_and_diag<T>(I, e, dx, f0, f1, f2, d);
+  if (I == 0 && live)
+    gs_node_update<T>(flags[blk * 64 + l], d, f0, f1, f2, blk, l, u[vidx(blk, 0, l)], u[vidx(blk, 1, l)],
+                      u[vidx(blk, 2, l)], b, u);
+}
+
+// colour order of the sweeps: nibble i = i-th colour
+struct ColourOrder {
+  unsigned v;
+  __host__ __device__ int at(int i) const { return (int)((v >> (4 * i)) & 7u); }
+};
+// boundary lists of one level, concatenated by colour
+struct BndLists {
+  const int* list;
+  int off[9];
+};
+
+static constexpr int BND_THREADS = 1024;  // fused boundary smoother: one workgroup
+
+// All boundary sweeps of one smoothing phase in a single workgroup: colour
+// steps are separated by workgroup barriers, which order the global-memory
+// updates between the waves of the workgroup (Multigrid_Hierarchy.h:577-586
+// boundary pre/post sweeps).
+template <class T>
+__global__ void __launch_bounds__(BND_THREADS) k_gs0_boundary(BndLists L, ColourOrder ord, int sweeps,
+                                                              const int* __restrict__ nbr,
+                                                              const uint8_t* __restrict__ flags,
+                                                              const T* __restrict__ E, T dx, T* __restrict__ u,
+                                                              const T* __restrict__ b) {
+  const int oct = threadIdx.x >> 3, I = threadIdx.x & 7;
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int ci = 0; ci < 8; ++ci) {
+      const int c = ord.at(ci);
+      const int n0 = L.off[c], n1 = L.off[c + 1];
+      for (int base = n0; base < n1; base += BND_THREADS / 8) {
+        const int i = base + oct;
+        const bool live = i < n1;
+        gs0_octet_global<T>(live, live ? L.list[i] : 0, I, nbr, flags, E, dx, u, b);
+      }
+      __syncthreads();
+    }
+}
+// one colour of a boundary list over many workgroups (long lists)
+template <class T>
+__global__ void __launch_bounds__(256) k_gs0_list(const int* __restrict__ list, int n, const int* __restrict__ nbr,
+                                                  const uint8_t* __restrict__ flags, const T* __restrict__ E, T dx,
+                                                  T* __restrict__ u, const T* __restrict__ b) {
+  const int i = blockIdx.x * 32 + (threadIdx.x >> 3);
+  const bool live = i < n;
+  gs0_octet_global<T>(live, live ? list[i] : 0, threadIdx.x & 7, nbr, flags, E, dx, u, b);
 }
 
 // --------------------------------------------------------- explicit levels
-__device__ inline int sidx(int blk, int e, int l) { return (blk * 243 + e) * 64 + l; }
+// Stencils of a (block, colour) are stored contiguously (st_index); lane
+// (k, g) = (lane>>3, lane&7) of an octet handles spokes g, g+8, g+16 and,
+// for g < 3, g+24 of colour node k, so each spoke group reads 64 consecutive
+// coefficients.  The spoke s = 13 (the 3x3 self block) falls to g = 5.
+template <class T, class LoadU>
+__device__ __forceinline__ void st_octet_row(int k, int g, const T* __restrict__ S, const LoadU& U, T& f0, T& f1,
+                                             T& f2, T c9[9]) {
+  T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t == 3 && g >= 3) break;
+    const int s = g + 8 * t;
+    T x, y, z;
+    U(s, x, y, z);
+    T m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] = t < 3 ? S[(t * 9 + q) * 64 + k * 8 + g] : S[1728 + q * 24 + k * 3 + g];
+    a0 += m[0] * x + m[1] * y + m[2] * z;
+    a1 += m[3] * x + m[4] * y + m[5] * z;
+    a2 += m[6] * x + m[7] * y + m[8] * z;
+    if (t == 1) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) c9[q] = m[q];
+    }
+  }
+  f0 = octet_sum(a0);
+  f1 = octet_sum(a1);
+  f2 = octet_sum(a2);
+}
+
+// MODE 0: one colour of the GS sweep over all blocks; MODE 1: residual
+// r = P(b - A u) over all (block, colour) waves (Stencil_Multiply).
+template <class T, int MODE>
+__global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0, const int* __restrict__ nbr,
+                                                             const uint8_t* __restrict__ flags,
+                                                             const T* __restrict__ ST, T* __restrict__ u,
+                                                             const T* __restrict__ b, T* __restrict__ r) {
+  __shared__ T hu[GS_WAVES][3][128];
+  __shared__ int ht[GS_WAVES][28];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wid = xcd_chunk(blockIdx.x, gridDim.x) * GS_WAVES + w;
+  const int blk = MODE == 0 ? wid : (wid >> 3);
+  const int colour = MODE == 0 ? colour0 : (wid & 7);
+  const bool live = blk < nb;
+  const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
+  if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  __syncthreads();
+  for (int h = lane; h < 125; h += 64) {
+    const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
+    const int bb = ht[w][ring_spoke(x, y, z)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+  }
+  __syncthreads();
+  const int k = lane >> 3, g = lane & 7;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;
+  const T* H = &hu[w][0][0];
+  auto U = [&](int s, T& x, T& y, T& z) {
+    const int h = ((nx + s / 9 - 1) * 5 + (ny + (s / 3) % 3 - 1)) * 5 + (nz + s % 3 - 1);
+    x = H[h];
+    y = H[128 + h];
+    z = H[256 + h];
+  };
+  const T* S = ST + ((int64_t)(live ? blk : 0) * 8 + colour) * 1944;
+  T f0, f1, f2, c9[9];
+  st_octet_row<T>(k, g, S, U, f0, f1, f2, c9);
+  if (g == 5 && live) {
+    const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+    const uint8_t fl = flags[blk * 64 + l];
+    if (MODE == 0) {
+      const int hn = (nx * 5 + ny) * 5 + nz;
+      T d[6] = {c9[0], c9[4], c9[8], c9[1], c9[2], c9[5]};
+      gs_node_update<T>(fl, d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], b, u);
+    } else {
+      const T f[3] = {f0, f1, f2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[vidx(blk, c, l)] = (fl & (2 << c)) ? T(0) : b[vidx(blk, c, l)] - f[c];
+    }
+  }
+}
 
 template <class T>
-__device__ inline void row_st(const int* __restrict__ nbr, const T* __restrict__ ST, const T* __restrict__ u, int blk,
-                              int l, T f[3]) {
+__device__ __forceinline__ void st_octet_global(bool live, int s, int g, const int* __restrict__ nbr,
+                                                const uint8_t* __restrict__ flags, const T* __restrict__ ST,
+                                                T* __restrict__ u, const T* __restrict__ b) {
+  const int blk = live ? (s >> 6) : 0, l = s & 63;
   const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
-  const Ring R = make_ring(nbr, blk, li, lj, lk);
-  T f0 = 0, f1 = 0, f2 = 0;
-#pragma unroll
-  for (int s = 0; s < 27; ++s) {
-    const int a = li + s / 9 - 1, b = lj + (s / 3) % 3 - 1, c = lk + s % 3 - 1;
-    const int bb = ring_block(R, a, b, c);
-    if (bb < 0) continue;
-    const int q = loc3(a, b, c);
-    const T x = u[vidx(bb, 0, q)], y = u[vidx(bb, 1, q)], z = u[vidx(bb, 2, q)];
-    const int e = s * 9;
-    f0 += ST[sidx(blk, e + 0, l)] * x + ST[sidx(blk, e + 1, l)] * y + ST[sidx(blk, e + 2, l)] * z;
-    f1 += ST[sidx(blk, e + 3, l)] * x + ST[sidx(blk, e + 4, l)] * y + ST[sidx(blk, e + 5, l)] * z;
-    f2 += ST[sidx(blk, e + 6, l)] * x + ST[sidx(blk, e + 7, l)] * y + ST[sidx(blk, e + 8, l)] * z;
-  }
-  f[0] = f0;
-  f[1] = f1;
-  f[2] = f2;
-}
-
-template <class T>
-__global__ void __launch_bounds__(TPB) k_st_residual(int nb, const int* __restrict__ nbr,
-                                                     const uint8_t* __restrict__ flags, const T* __restrict__ ST,
-                                                     const T* __restrict__ u, const T* __restrict__ b,
-                                                     T* __restrict__ r) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nb * 64) return;
-  const int blk = gid >> 6, l = gid & 63;
-  T f[3];
-  row_st(nbr, ST, u, blk, l, f);
-  const uint8_t fl = flags[gid];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T v = b[vidx(blk, c, l)] - f[c];
-    r[vidx(blk, c, l)] = (fl & (2 << c)) ? T(0) : v;
+  const int colour = ((li & 1) << 2) | ((lj & 1) << 1) | (lk & 1);
+  const int k = ((li >> 1) << 2) | ((lj >> 1) << 1) | (lk >> 1);
+  const int* t = nbr + blk * 27;
+  auto U = [&](int sp, T& x, T& y, T& z) {
+    const int a = li + sp / 9 - 1, bq = lj + (sp / 3) % 3 - 1, c = lk + sp % 3 - 1;
+    const int rs = ring_spoke(a, bq, c);
+    const int bb = !live ? -1 : (rs == 13 ? blk : t[rs]);
+    const int q = loc3(a, bq, c);
+    x = bb >= 0 ? u[vidx(bb, 0, q)] : T(0);
+    y = bb >= 0 ? u[vidx(bb, 1, q)] : T(0);
+    z = bb >= 0 ? u[vidx(bb, 2, q)] : T(0);
+  };
+  const T* S = ST + ((int64_t)blk * 8 + colour) * 1944;
+  T f0, f1, f2, c9[9];
+  st_octet_row<T>(k, g, S, U, f0, f1, f2, c9);
+  if (g == 5 && live) {
+    T d[6] = {c9[0], c9[4], c9[8], c9[1], c9[2], c9[5]};
+    gs_node_update<T>(flags[blk * 64 + l], d, f0, f1, f2, blk, l, u[vidx(blk, 0, l)], u[vidx(blk, 1, l)],
+                      u[vidx(blk, 2, l)], b, u);
   }
 }
 
 template <class T>
-__global__ void __launch_bounds__(TPB) k_st_gs(int nb, int colour, const int* __restrict__ nbr,
-                                               const uint8_t* __restrict__ flags, const T* __restrict__ ST, T* u,
-                                               const T* __restrict__ b, const int* __restrict__ list, int nlist) {
+__global__ void __launch_bounds__(BND_THREADS) k_st_boundary(BndLists L, ColourOrder ord, int sweeps,
+                                                             const int* __restrict__ nbr,
+                                                             const uint8_t* __restrict__ flags,
+                                                             const T* __restrict__ ST, T* __restrict__ u,
+                                                             const T* __restrict__ b) {
+  const int oct = threadIdx.x >> 3, g = threadIdx.x & 7;
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int ci = 0; ci < 8; ++ci) {
+      const int c = ord.at(ci);
+      const int n0 = L.off[c], n1 = L.off[c + 1];
+      for (int base = n0; base < n1; base += BND_THREADS / 8) {
+        const int i = base + oct;
+        const bool live = i < n1;
+        st_octet_global<T>(live, live ? L.list[i] : 0, g, nbr, flags, ST, u, b);
+      }
+      __syncthreads();
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_st_list(const int* __restrict__ list, int n, const int* __restrict__ nbr,
+                                                 const uint8_t* __restrict__ flags, const T* __restrict__ ST,
+                                                 T* __restrict__ u, const T* __restrict__ b) {
+  const int i = blockIdx.x * 32 + (threadIdx.x >> 3);
+  const bool live = i < n;
+  st_octet_global<T>(live, live ? list[i] : 0, threadIdx.x & 7, nbr, flags, ST, u, b);
+}
+
+// ---------------------------------------------- LDS-resident boundary phase
+// Boundary nodes are few (the 3 rings around Dirichlet nodes) and only they
+// change during a boundary phase, so the phase runs on a compact system:
+// b~ = b - (row part of the nodes outside the boundary set) is formed once,
+// then every colour step of every sweep updates the boundary u held in LDS.
+// Coefficient rows: A[i*288 + t*72 + m*8 + g] is entry m = v*3+w of the 3x3
+// block of spoke s = g + 8t of boundary node i (zero for s >= 27).
+
+// level 0: A_s = dx * sum over the cells holding the node (corner I) and its
+// neighbour (corner I + o) of E_cell K0[I][I+o] (the rows of the
+// matrix-free operator, SPGrid_Linear_Elasticity.h:22-94)
+__global__ void k_bnd_coef0(int n, const int* __restrict__ list, const int* __restrict__ nbr,
+                            const double* __restrict__ E, double dx, double* __restrict__ A) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  int blk, l;
-  if (list) {
-    if (gid >= nlist) return;
-    const int s = list[gid];
-    blk = s >> 6;
-    l = s & 63;
-  } else {
-    if (gid >= nb * 8) return;
-    blk = gid >> 3;
-    l = colour_node(colour, gid & 7);
+  if (gid >= n * 32) return;
+  const int i = gid >> 5, t = (gid >> 3) & 3, g = gid & 7, s = g + 8 * t;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < 27) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+    const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
+    const int ox = s / 9 - 1, oy = (s / 3) % 3 - 1, oz = s % 3 - 1;
+    for (int I = 0; I < 8; ++I) {
+      const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
+      const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+      if (jx < 0 || jx > 1 || jy < 0 || jy > 1 || jz < 0 || jz > 1) continue;
+      const int cx = li - ix, cy = lj - iy, cz = lk - iz;
+      const int rs = ring_spoke(cx, cy, cz);
+      const int cb = rs == 13 ? blk : nbr[blk * 27 + rs];
+      if (cb < 0) continue;
+      const double e = E[cb * 64 + loc3(cx, cy, cz)];
+      const int kb = (I * 8 + ((jx << 2) | (jy << 1) | jz)) * 9;
+      for (int m = 0; m < 9; ++m) acc[m] += e * cK0d[kb + m];
+    }
   }
-  const uint8_t fl = flags[blk * 64 + l];
-  if (fl & 1) {
-    T f[3];
-    row_st(nbr, ST, u, blk, l, f);
-    const int e = 13 * 9;
-    T d[6] = {ST[sidx(blk, e + 0, l)], ST[sidx(blk, e + 4, l)], ST[sidx(blk, e + 8, l)],
-              ST[sidx(blk, e + 1, l)], ST[sidx(blk, e + 2, l)], ST[sidx(blk, e + 5, l)]};
-    dirichlet_inverse(d, fl);
-    const T r0 = b[vidx(blk, 0, l)] - f[0], r1 = b[vidx(blk, 1, l)] - f[1], r2 = b[vidx(blk, 2, l)] - f[2];
-    const T u0 = u[vidx(blk, 0, l)] + d[0] * r0 + d[3] * r1 + d[4] * r2;
-    const T u1 = u[vidx(blk, 1, l)] + d[3] * r0 + d[1] * r1 + d[5] * r2;
-    const T u2 = u[vidx(blk, 2, l)] + d[4] * r0 + d[5] * r1 + d[2] * r2;
-    u[vidx(blk, 0, l)] = (fl & 2) ? T(0) : u0;
-    u[vidx(blk, 1, l)] = (fl & 4) ? T(0) : u1;
-    u[vidx(blk, 2, l)] = (fl & 8) ? T(0) : u2;
-  } else if (fl & 0x0E) {
+  for (int m = 0; m < 9; ++m) A[(size_t)i * 288 + t * 72 + m * 8 + g] = dx * acc[m];
+}
+// explicit levels: the Galerkin stencil rows
+__global__ void k_bnd_coef_st(int n, const int* __restrict__ list, const double* __restrict__ ST,
+                              double* __restrict__ A) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n * 32) return;
+  const int i = gid >> 5, t = (gid >> 3) & 3, g = gid & 7, s = g + 8 * t;
+  const int node = list[i];
+  for (int m = 0; m < 9; ++m)
+    A[(size_t)i * 288 + t * 72 + m * 8 + g] = s < 27 ? ST[st_index(node >> 6, node & 63, s * 9 + m)] : 0.0;
+}
+
+// b~_i = b_i - sum over spokes to nodes outside the boundary set
+template <class T>
+__global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ list, const int* __restrict__ bslot,
+                                                 const T* __restrict__ A, const T* __restrict__ u,
+                                                 const T* __restrict__ b, T* __restrict__ bt) {
+  const int i = blockIdx.x * 32 + (threadIdx.x >> 3), g = threadIdx.x & 7;
+  const bool live = i < n;
+  T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      if (fl & (2 << c)) u[vidx(blk, c, l)] = T(0);
+  for (int t = 0; t < 4; ++t) {
+    if (t == 3 && g >= 3) break;
+    const int sl = live ? bslot[i * 32 + t * 8 + g] : -1;
+    if (sl >= 0) {
+      const T x = u[vidx(sl >> 6, 0, sl & 63)], y = u[vidx(sl >> 6, 1, sl & 63)], z = u[vidx(sl >> 6, 2, sl & 63)];
+      const T* M = A + (size_t)i * 288 + t * 72 + g;
+      a0 += M[0] * x + M[8] * y + M[16] * z;
+      a1 += M[24] * x + M[32] * y + M[40] * z;
+      a2 += M[48] * x + M[56] * y + M[64] * z;
+    }
+  }
+  a0 = octet_sum(a0);
+  a1 = octet_sum(a1);
+  a2 = octet_sum(a2);
+  if (g == 0 && live) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+    bt[i * 3 + 0] = b[vidx(blk, 0, l)] - a0;
+    bt[i * 3 + 1] = b[vidx(blk, 1, l)] - a1;
+    bt[i * 3 + 2] = b[vidx(blk, 2, l)] - a2;
   }
 }
+
+struct BndOff {
+  int off[9];
+};
+// largest dynamic LDS of one workgroup on gfx950
+static constexpr size_t kMaxLds = 163840;
+
+// Per octet and pass: the coefficient row (36 values per lane), the
+// neighbour indices of the lane's spokes and b~ of the node.  Loaded one pass
+// ahead so that global-memory latency overlaps the previous pass.
+template <class T>
+struct BndFetch {
+  T m[4][9];
+  int j[4];
+  T r[3];
+  int i;
+  bool live;
+};
+template <class T>
+__device__ __forceinline__ void bnd_fetch(BndFetch<T>& F, int i, bool live, int g, const int* __restrict__ bci,
+                                          const T* __restrict__ A, const T* __restrict__ bt) {
+  F.i = i;
+  F.live = live;
+  const T* M = A + (size_t)(live ? i : 0) * 288 + g;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const bool has = live && (t < 3 || g < 3);
+    F.j[t] = has ? bci[i * 32 + t * 8 + g] : -1;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) F.m[t][q] = has ? M[t * 72 + q * 8] : T(0);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F.r[c] = (live && g == 5) ? bt[i * 3 + c] : T(0);
+}
+
+template <class T, int NT>
+__global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder ord, int sweeps,
+                                                   const int* __restrict__ list, const int* __restrict__ bci,
+                                                   const T* __restrict__ A, const uint8_t* __restrict__ flags,
+                                                   const T* __restrict__ bt, T* __restrict__ u) {
+  constexpr int NO = NT / 8;  // octets = nodes per pass
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* ub = reinterpret_cast<T*>(smem_raw);                 // [3][n]
+  uint8_t* fb = reinterpret_cast<uint8_t*>(ub + 3 * n);  // [n]
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ub[c * n + i] = u[vidx(blk, c, l)];
+    fb[i] = flags[node];
+  }
+  const int oct = threadIdx.x >> 3, g = threadIdx.x & 7;
+  int sw = 0, ci = 0, p = 0, c = ord.at(0);
+  BndFetch<T> cur, nxt;
+  {
+    const int i = L.off[c] + oct;
+    bnd_fetch<T>(cur, i, i < L.off[c + 1], g, bci, A, bt);
+  }
+  __syncthreads();
+  while (true) {
+    // the pass after this one
+    int sw2 = sw, ci2 = ci, p2 = p + 1, c2 = c;
+    const bool step_end = L.off[c] + p2 * NO >= L.off[c + 1];
+    if (step_end) {
+      p2 = 0;
+      if (++ci2 == 8) {
+        ci2 = 0;
+        ++sw2;
+      }
+      c2 = ord.at(ci2);
+    }
+    const bool more = sw2 < sweeps;
+    if (more) {
+      const int i2 = L.off[c2] + p2 * NO + oct;
+      bnd_fetch<T>(nxt, i2, i2 < L.off[c2 + 1], g, bci, A, bt);
+    }
+    // this pass: row of node cur.i over the boundary neighbours (LDS)
+    T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int j = cur.j[t];
+      if (j >= 0) {
+        const T x = ub[j], y = ub[n + j], z = ub[2 * n + j];
+        a0 += cur.m[t][0] * x + cur.m[t][1] * y + cur.m[t][2] * z;
+        a1 += cur.m[t][3] * x + cur.m[t][4] * y + cur.m[t][5] * z;
+        a2 += cur.m[t][6] * x + cur.m[t][7] * y + cur.m[t][8] * z;
+      }
+    }
+    a0 = octet_sum(a0);
+    a1 = octet_sum(a1);
+    a2 = octet_sum(a2);
+    if (g == 5 && cur.live) {
+      const int i = cur.i;
+      const uint8_t fl = fb[i];
+      if (fl & 1) {
+        // spoke 13 (the node itself) is group t = 1 of lane g = 5
+        T d[6] = {cur.m[1][0], cur.m[1][4], cur.m[1][8], cur.m[1][1], cur.m[1][2], cur.m[1][5]};
+        dirichlet_inverse(d, fl);
+        const T r0 = cur.r[0] - a0, r1 = cur.r[1] - a1, r2 = cur.r[2] - a2;
+        const T u0 = ub[i] + d[0] * r0 + d[3] * r1 + d[4] * r2;
+        const T u1 = ub[n + i] + d[3] * r0 + d[1] * r1 + d[5] * r2;
+        const T u2 = ub[2 * n + i] + d[4] * r0 + d[5] * r1 + d[2] * r2;
+        ub[i] = (fl & 2) ? T(0) : u0;
+        ub[n + i] = (fl & 4) ? T(0) : u1;
+        ub[2 * n + i] = (fl & 8) ? T(0) : u2;
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+          if (fl & (2 << cc)) ub[cc * n + i] = T(0);
+      }
+    }
+    if (step_end) __syncthreads();
+    if (!more) break;
+    cur = nxt;
+    sw = sw2;
+    ci = ci2;
+    p = p2;
+    c = c2;
+  }
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n + i];
+  }
+}
+// workgroup size of the boundary phase: two register sets of BndFetch per lane
+template <class T>
+struct BndThreads {
+  static constexpr int value = sizeof(T) == 4 ? 1024 : 512;
+};
 
 // ----------------------------------------------------------------- transfers
 // R = P^T, full weighting {1/2,1,1/2}^3 around 2c-B (Restriction.h:52-92)
@@ -568,7 +1104,7 @@ __global__ void __launch_bounds__(TPB) k_galerkin_assemble(int nb, const int* __
         if (!(fl & 1)) val = 0.0;
         else if (fl & (2 << v)) val = (v == w && s == 13) ? 1.0 : 0.0;
         else if (mfl & (2 << w)) val = 0.0;
-        ST[sidx(blk, s * 9 + v * 3 + w, l)] = val;
+        ST[st_index(blk, l, s * 9 + v * 3 + w)] = val;
       }
   }
 }
@@ -591,7 +1127,7 @@ __global__ void k_dense_assemble(int nb, const int* __restrict__ nbr, const uint
     const int vj = var_of[bb * 64 + loc3(a, b, c)];
     if (vj < 0) continue;
     for (int v = 0; v < 3; ++v)
-      for (int w = 0; w < 3; ++w) A[(int64_t)(vi + v) + (int64_t)(vj + w) * n] = ST[sidx(blk, s * 9 + v * 3 + w, l)];
+      for (int w = 0; w < 3; ++w) A[(int64_t)(vi + v) + (int64_t)(vj + w) * n] = ST[st_index(blk, l, s * 9 + v * 3 + w)];
   }
 }
 __global__ void k_symmetrize_lower(double* A, int64_t n) {
@@ -601,22 +1137,85 @@ __global__ void k_symmetrize_lower(double* A, int64_t n) {
   if (i < j) A[i + j * n] = A[j + i * n];
 }
 template <class T>
-__global__ void k_bottom_gather(int nb, const int* __restrict__ var_of, const T* __restrict__ b, double* x) {
+__global__ void k_bottom_gather(int nb, const int* __restrict__ var_of, const T* __restrict__ b, T* x) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nb * 64) return;
   const int vi = var_of[gid];
   if (vi < 0) return;
   const int blk = gid >> 6, l = gid & 63;
-  for (int c = 0; c < 3; ++c) x[vi + c] = (double)b[vidx(blk, c, l)];
+  for (int c = 0; c < 3; ++c) x[vi + c] = b[vidx(blk, c, l)];
 }
 template <class T>
-__global__ void k_bottom_scatter(int nb, const int* __restrict__ var_of, const double* y, const T* __restrict__ b,
-                                 T* u) {
+__global__ void k_bottom_scatter(int nb, const int* __restrict__ var_of, const T* y, const T* __restrict__ b, T* u) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nb * 64) return;
   const int vi = var_of[gid];
   const int blk = gid >> 6, l = gid & 63;
-  for (int c = 0; c < 3; ++c) u[vidx(blk, c, l)] = vi >= 0 ? (T)y[vi + c] : b[vidx(blk, c, l)];
+  for (int c = 0; c < 3; ++c) u[vidx(blk, c, l)] = vi >= 0 ? y[vi + c] : b[vidx(blk, c, l)];
+}
+
+// The symmetric bottom inverse kept as its lower triangle in 64x64 tiles:
+// tile (I, J), I >= J, at index I(I+1)/2 + J, row-major, zero padded.
+template <class T>
+__global__ void k_pack_sym_tiles(const double* __restrict__ A, int64_t nv, int nT, T* __restrict__ tiles) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ntile = (int64_t)nT * (nT + 1) / 2;
+  if (gid >= ntile * 4096) return;
+  const int64_t tile = gid >> 12;
+  const int e = (int)(gid & 4095);
+  int I = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((int64_t)(I + 1) * (I + 2) / 2 <= tile) ++I;
+  while ((int64_t)I * (I + 1) / 2 > tile) --I;
+  const int J = (int)(tile - (int64_t)I * (I + 1) / 2);
+  const int64_t row = (int64_t)I * 64 + (e >> 6), col = (int64_t)J * 64 + (e & 63);
+  tiles[gid] = (row < nv && col < nv) ? (T)A[row + col * nv] : T(0);
+}
+
+// y = A x with the packed tiles.  Workgroup (I, s) takes the tile columns
+// J = s, s+S, ... of row block I: a stored tile (J <= I) contributes
+// T(I,J) x_J, a mirrored one (J > I) contributes T(J,I)^T x_J.  Partial
+// sums per s are added in a fixed order by k_symv_reduce (deterministic).
+template <class T>
+__global__ void __launch_bounds__(256) k_symv_tiles(int nT, int S, const T* __restrict__ tiles,
+                                                    const T* __restrict__ x, T* __restrict__ part) {
+  const int I = blockIdx.x, s = blockIdx.y;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  T accN[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) accN[k] = 0;
+  T accT = 0;
+  for (int J = s; J < nT; J += S) {
+    if (J <= I) {
+      const T* P = tiles + ((int64_t)I * (I + 1) / 2 + J) * 4096;
+      const T xv = x[J * 64 + lane];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) accN[k] += P[(w + 4 * k) * 64 + lane] * xv;
+    } else {
+      const T* P = tiles + ((int64_t)J * (J + 1) / 2 + I) * 4096;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) accT += P[(w + 4 * k) * 64 + lane] * x[J * 64 + w + 4 * k];
+    }
+  }
+  __shared__ T ys[64];
+  __shared__ T yt[4][64];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    T v = accN[k];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) ys[w + 4 * k] = v;
+  }
+  yt[w][lane] = accT;
+  __syncthreads();
+  if (t < 64) part[((int64_t)s * nT + I) * 64 + t] = ys[t] + yt[0][t] + yt[1][t] + yt[2][t] + yt[3][t];
+}
+template <class T>
+__global__ void k_symv_reduce(int n, int S, int64_t stride, const T* __restrict__ part, T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a = 0;
+  for (int s = 0; s < S; ++s) a += part[s * stride + i];
+  y[i] = a;
 }
 
 // -------------------------------------------------------------- sensitivity
@@ -697,36 +1296,127 @@ __global__ void k_set_entries(double* v, const int* idx, const double* val, int6
 // ======================================================== launch wrappers
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {
+  const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
   if (mode == 0)
-    k_apply<T, 0><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, 0><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
   else if (mode == 2)
-    k_apply<T, 2><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, 2><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
   else
-    k_apply<T, 1><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+}
+static ColourOrder colour_order(bool level0, bool forward) {
+  ColourOrder o{0};
+  for (int i = 0; i < 8; ++i) o.v |= (unsigned)sweep_colour(level0, forward, i) << (4 * i);
+  return o;
+}
+static BndLists bnd_lists(const LevelDev& L) {
+  BndLists bl;
+  bl.list = L.bnd;
+  for (int c = 0; c < 9; ++c) bl.off[c] = L.bnd_off[c];
+  return bl;
+}
+// a boundary phase runs in one workgroup while no colour list is longer than this
+static constexpr int kFusedBoundaryMax = 4096;
+// SPG_BOUNDARY_PATH selects the boundary smoother for tests: 0 (default)
+// automatic, 1 the single-workgroup global-memory kernel, 2 one launch per
+// colour step.  Paths 1 and 2 are the fallbacks for boundary sets too large
+// for the LDS-resident system.
+static int boundary_path() {
+  const char* e = std::getenv("SPG_BOUNDARY_PATH");
+  return e ? std::atoi(e) : 0;
+}
+
+template <class T>
+struct BndPtr;
+template <>
+struct BndPtr<float> {
+  static const float* A(const LevelDev& L) { return L.bA32; }
+  static float* bt(const LevelDev& L) { return L.bt32; }
+};
+template <>
+struct BndPtr<double> {
+  static const double* A(const LevelDev& L) { return L.bA64; }
+  static double* bt(const LevelDev& L) { return L.bt64; }
+};
+// the boundary phase on the LDS-resident system; false if it does not fit
+template <class T>
+static bool launch_bnd_phase(hipStream_t st, const LevelDev& L, T* u, const T* b, int sweeps, bool level0,
+                             bool forward) {
+  const int n = L.bnd_off[8];
+  const size_t lds = (size_t)3 * n * sizeof(T) + (size_t)n;
+  const T* A = BndPtr<T>::A(L);
+  T* bt = BndPtr<T>::bt(L);
+  if (!A || !bt || !L.bci || !L.bslot || lds > kMaxLds) return false;
+  constexpr int NT = BndThreads<T>::value;
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute((const void*)k_bnd_sweeps<T, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxLds);
+    return true;
+  }();
+  (void)attr;
+  k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt);
+  BndOff o;
+  for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
+  k_bnd_sweeps<T, NT><<<1, NT, lds, st>>>(n, o, colour_order(level0, forward), sweeps, L.bnd, L.bci, A, L.flags, bt,
+                                          u);
+  return true;
+}
+void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double dx, double* A) {
+  const int n = L.bnd_off[8];
+  if (n > 0) k_bnd_coef0<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, L.nbr, E, dx, A);
+}
+void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A) {
+  const int n = L.bnd_off[8];
+  if (n > 0) k_bnd_coef_st<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, ST, A);
+}
+
+template <class T>
+void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
+  const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
+  k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
 }
 template <class T>
-void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour, bool boundary) {
-  if (boundary) {
-    const int n = L.nbnd[colour];
-    if (n == 0) return;
-    k_gs0<T><<<nblk(n), TPB, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, L.bnd[colour], n);
-  } else {
-    k_gs0<T><<<nblk((int64_t)L.nb * 8), TPB, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr, 0);
+void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
+                         bool forward) {
+  if (sweeps <= 0 || L.bnd_off[8] == 0) return;
+  const int path = boundary_path();
+  if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
+  const ColourOrder ord = colour_order(true, forward);
+  if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {
+    k_gs0_boundary<T><<<1, BND_THREADS, 0, st>>>(bnd_lists(L), ord, sweeps, L.nbr, L.flags, E, dx, u, b);
+    return;
   }
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int i = 0; i < 8; ++i) {
+      const int c = ord.at(i), n = L.bnd_off[c + 1] - L.bnd_off[c];
+      if (n > 0) k_gs0_list<T><<<(n + 31) / 32, 256, 0, st>>>(L.bnd + L.bnd_off[c], n, L.nbr, L.flags, E, dx, u, b);
+    }
 }
 template <class T>
 void launch_st_residual(hipStream_t st, const LevelDev& L, const T* ST, const T* u, const T* b, T* r) {
-  k_st_residual<T><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, ST, u, b, r);
+  const int nwg = (L.nb * 8 + GS_WAVES - 1) / GS_WAVES;
+  k_st_colour<T, 1><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, 0, L.nbr, L.flags, ST, const_cast<T*>(u), b, r);
 }
 template <class T>
-void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int colour, bool boundary) {
-  if (boundary) {
-    const int n = L.nbnd[colour];
-    if (n == 0) return;
-    k_st_gs<T><<<nblk(n), TPB, 0, st>>>(L.nb, colour, L.nbr, L.flags, ST, u, b, L.bnd[colour], n);
-  } else {
-    k_st_gs<T><<<nblk((int64_t)L.nb * 8), TPB, 0, st>>>(L.nb, colour, L.nbr, L.flags, ST, u, b, nullptr, 0);
+void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int colour) {
+  const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
+  k_st_colour<T, 0><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, ST, u, b, nullptr);
+}
+template <class T>
+void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int sweeps, bool forward) {
+  if (sweeps <= 0 || L.bnd_off[8] == 0) return;
+  const int path = boundary_path();
+  if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, false, forward)) return;
+  const ColourOrder ord = colour_order(false, forward);
+  if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {
+    k_st_boundary<T><<<1, BND_THREADS, 0, st>>>(bnd_lists(L), ord, sweeps, L.nbr, L.flags, ST, u, b);
+    return;
   }
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int i = 0; i < 8; ++i) {
+      const int c = ord.at(i), n = L.bnd_off[c + 1] - L.bnd_off[c];
+      if (n > 0) k_st_list<T><<<(n + 31) / 32, 256, 0, st>>>(L.bnd + L.bnd_off[c], n, L.nbr, L.flags, ST, u, b);
+    }
 }
 template <class T>
 void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse) {
@@ -806,12 +1496,24 @@ void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n) {
   k_symmetrize_lower<<<nblk(n * n), TPB, 0, st>>>(A, n);
 }
 template <class T>
-void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, double* x) {
+void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, T* x) {
   k_bottom_gather<T><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, var_of, b, x);
 }
 template <class T>
-void launch_bottom_scatter(hipStream_t st, const LevelDev& L, const int* var_of, const double* y, const T* b, T* u) {
+void launch_bottom_scatter(hipStream_t st, const LevelDev& L, const int* var_of, const T* y, const T* b, T* u) {
   k_bottom_scatter<T><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, var_of, y, b, u);
+}
+template <class T>
+void launch_pack_sym_tiles(hipStream_t st, const double* A, int64_t nv, int nT, T* tiles) {
+  const int64_t n = (int64_t)nT * (nT + 1) / 2 * 4096;
+  k_pack_sym_tiles<T><<<nblk(n), TPB, 0, st>>>(A, nv, nT, tiles);
+}
+int symv_splits(int nT) { return std::max(1, std::min(nT, 2048 / std::max(nT, 1))); }
+template <class T>
+void launch_symv(hipStream_t st, int nT, const T* tiles, const T* x, T* part, T* y) {
+  const int S = symv_splits(nT);
+  k_symv_tiles<T><<<dim3(nT, S), 256, 0, st>>>(nT, S, tiles, x, part);
+  k_symv_reduce<T><<<nblk((int64_t)nT * 64), TPB, 0, st>>>(nT * 64, S, (int64_t)nT * 64, part, y);
 }
 void launch_sensitivity(hipStream_t st, const LevelDev& L, const double* rho, const double* u, const double* a,
                         double penalty, double emin, double* dc, double* objcell) {
@@ -832,9 +1534,11 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
 
 #define SPG_INST(T)                                                                                              \
   template void launch_apply<T>(hipStream_t, const LevelDev&, const T*, T, const T*, const T*, T*, int);         \
-  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);               \
+  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int);                     \
+  template void launch_gs0_boundary<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);      \
   template void launch_st_residual<T>(hipStream_t, const LevelDev&, const T*, const T*, const T*, T*);           \
-  template void launch_st_gs<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int, bool);                \
+  template void launch_st_gs<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int);                      \
+  template void launch_st_boundary<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int, bool);          \
   template void launch_restrict<T>(hipStream_t, const LevelDev&, const T*, T*);                                  \
   template void launch_prolongate<T>(hipStream_t, const LevelDev&, const T*, T*);                                \
   template void launch_zero<T>(hipStream_t, T*, int64_t);                                                        \
@@ -845,8 +1549,10 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_cg_direction<T>(hipStream_t, T*, const T*, const CgScalars*, int64_t);                    \
   template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, const CgScalars*, int64_t);         \
   template void launch_project<T>(hipStream_t, const LevelDev&, T*, bool);                                       \
-  template void launch_bottom_gather<T>(hipStream_t, const LevelDev&, const int*, const T*, double*);            \
-  template void launch_bottom_scatter<T>(hipStream_t, const LevelDev&, const int*, const double*, const T*, T*);
+  template void launch_bottom_gather<T>(hipStream_t, const LevelDev&, const int*, const T*, T*);                 \
+  template void launch_bottom_scatter<T>(hipStream_t, const LevelDev&, const int*, const T*, const T*, T*);      \
+  template void launch_pack_sym_tiles<T>(hipStream_t, const double*, int64_t, int, T*);                          \
+  template void launch_symv<T>(hipStream_t, int, const T*, const T*, T*, T*);
 SPG_INST(float)
 SPG_INST(double)
 

@@ -2,8 +2,9 @@
 //
 // Level data layout (device): blocks of 4x4x4 nodes, local index
 // l = i*16 + j*4 + k; a vector is stored per block and component:
-// v[(blk*3 + c)*64 + l]; explicit-level stencils ST[(blk*243 + e)*64 + l]
-// with e = ((s*3 + v)*3 + w), s = (di+1)*9 + (dj+1)*3 + (dk+1).
+// v[(blk*3 + c)*64 + l].  Explicit-level stencils hold 243 coefficients
+// per node, e = ((s*3 + v)*3 + w), s = (di+1)*9 + (dj+1)*3 + (dk+1), grouped
+// by (block, colour) as st_index() describes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,9 +19,40 @@ struct LevelDev {
   const int* rsrc = nullptr;       // [nb*27] restriction sources (level >= 1)
   const int* psrc = nullptr;       // [nb*8]  prolongation sources (level < L-1)
   const int* base = nullptr;       // [nb*3]  block origin (node coords)
-  const int* bnd[8] = {};          // boundary nodes per colour
-  int nbnd[8] = {};
+  const int* bnd = nullptr;        // boundary nodes (blk*64+l), concatenated by colour
+  int bnd_off[9] = {};             // colour c owns bnd[bnd_off[c] .. bnd_off[c+1])
+  int bnd_max = 0;                 // longest colour list
+  // boundary system (LDS-resident boundary phase): neighbour tables [n*32]
+  // and coefficient rows [n*288] per precision, plus the b~ scratch [n*3]
+  const int* bci = nullptr;
+  const int* bslot = nullptr;
+  const float* bA32 = nullptr;
+  const double* bA64 = nullptr;
+  float* bt32 = nullptr;
+  double* bt64 = nullptr;
 };
+
+// Colour of the i-th step of a Gauss-Seidel sweep: level 0 runs the colours
+// 0..7 forward / 7..0 backward (SPGrid_Gauss_Seidel.h:765-875); the
+// explicit levels use the reference's colour order
+// (Multigrid_Hierarchy.h:577-617).
+__host__ __device__ inline int sweep_colour(bool level0, bool forward, int i) {
+  const unsigned ord = level0 ? (forward ? 0x76543210u : 0x01234567u) : (forward ? 0x02376451u : 0x15467320u);
+  return (int)((ord >> (4 * i)) & 7u);
+}
+
+// Explicit-level stencil layout.  The 1944 coefficients of the 8 nodes of one
+// colour in one block are contiguous; node k (0..7 inside the colour) and
+// spoke s = g + 8t go to lane k*8+g of row t*9+m (t < 3) or, for the spokes
+// 24..26, to the tail 1728 + m*24 + k*3 + g (m = v*3+w).
+__host__ __device__ inline int64_t st_index(int64_t blk, int l, int e) {
+  const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
+  const int colour = ((li & 1) << 2) | ((lj & 1) << 1) | (lk & 1);
+  const int k = ((li >> 1) << 2) | ((lj >> 1) << 1) | (lk >> 1);
+  const int s = e / 9, m = e % 9, g = s & 7, t = s >> 3;
+  const int64_t base = (blk * 8 + colour) * 1944;
+  return t < 3 ? base + (t * 9 + m) * 64 + k * 8 + g : base + 1728 + m * 24 + k * 3 + g;
+}
 
 // device scalars of one PCG iteration
 struct CgScalars {
@@ -34,14 +66,19 @@ void upload_k0(const double* K0, hipStream_t st);
 // level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode);
-// level-0 GS: one colour over all blocks (boundary=false) or over the boundary list
+// level-0 GS: one colour over all blocks; all boundary sweeps of one phase
 template <class T>
-void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour, bool boundary);
+void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour);
+template <class T>
+void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
+                         bool forward);
 // explicit levels
 template <class T>
 void launch_st_residual(hipStream_t st, const LevelDev& L, const T* ST, const T* u, const T* b, T* r);
 template <class T>
-void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int colour, bool boundary);
+void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int colour);
+template <class T>
+void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int sweeps, bool forward);
 // transfers
 template <class T>
 void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse);
@@ -78,14 +115,26 @@ void launch_galerkin_gather(hipStream_t st, int64_t ncell, int depth, const int*
 void launch_galerkin_assemble(hipStream_t st, const LevelDev& L, const int* cell_row, const double* Cm, double dx,
                               double* ST);
 void launch_stencil_to_float(hipStream_t st, const double* a, float* b, int64_t n);
+// coefficient rows of the boundary nodes (FP64): level 0 from E and K0 (times
+// dx), explicit levels from the stencils
+void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double dx, double* A);
+void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A);
 // bottom
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
                            double* A);
 void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n);
 template <class T>
-void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, double* x);
+void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, T* x);
 template <class T>
-void launch_bottom_scatter(hipStream_t st, const LevelDev& L, const int* var_of, const double* y, const T* b, T* u);
+void launch_bottom_scatter(hipStream_t st, const LevelDev& L, const int* var_of, const T* y, const T* b, T* u);
+// symmetric matrix in packed lower-triangle 64x64 tiles: pack from a dense
+// column-major FP64 matrix, and y = A x (x, y padded to nT*64; part holds
+// symv_splits(nT) * nT * 64 partial sums)
+template <class T>
+void launch_pack_sym_tiles(hipStream_t st, const double* A, int64_t nv, int nT, T* tiles);
+int symv_splits(int nT);
+template <class T>
+void launch_symv(hipStream_t st, int nT, const T* tiles, const T* x, T* part, T* y);
 // sensitivity: per level-0 slot (cell): dc = -p rho^(p-1)(1-Emin) a_e^T K0 u_e ; obj partials
 void launch_sensitivity(hipStream_t st, const LevelDev& L, const double* rho, const double* u, const double* a,
                         double penalty, double emin, double* dc, double* objcell);

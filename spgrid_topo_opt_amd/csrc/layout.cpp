@@ -346,6 +346,32 @@ Layout build_layout(const ProblemDesc& p) {
       for (int s = 0; s < 27; ++s) D.nbr[g * 27 + s] = gfind(bx + s / 9 - 1, by + (s / 3) % 3 - 1, bz + s % 3 - 1);
     }
   }
+  // boundary neighbour tables
+  for (int l = 0; l < out.nlev; ++l) {
+    LevelDesc& D = out.lv[l];
+    std::vector<int32_t> all;
+    for (int c = 0; c < 8; ++c) all.insert(all.end(), D.bnd_list[c].begin(), D.bnd_list[c].end());
+    std::unordered_map<int32_t, int32_t> pos;
+    pos.reserve(all.size() * 2);
+    for (size_t i = 0; i < all.size(); ++i) pos.emplace(all[i], (int32_t)i);
+    D.bnd_ci.assign(all.size() * 32, -1);
+    D.bnd_slot.assign(all.size() * 32, -1);
+    for (size_t i = 0; i < all.size(); ++i) {
+      const int32_t g = all[i] >> 6, q = all[i] & 63;
+      const int li = q >> 4, lj = (q >> 2) & 3, lk = q & 3;
+      for (int sp = 0; sp < 27; ++sp) {
+        const int a = li + sp / 9 - 1, b = lj + (sp / 3) % 3 - 1, c = lk + sp % 3 - 1;
+        const int rs = (a < 0 ? 0 : (a > 3 ? 2 : 1)) * 9 + (b < 0 ? 0 : (b > 3 ? 2 : 1)) * 3 + (c < 0 ? 0 : (c > 3 ? 2 : 1));
+        const int32_t nb = D.nbr[(size_t)g * 27 + rs];
+        if (nb < 0) continue;
+        const int32_t slot = nb * 64 + (((a & 3) << 4) | ((b & 3) << 2) | (c & 3));
+        const size_t e = i * 32 + (size_t)(sp >> 3) * 8 + (sp & 7);
+        auto it = pos.find(slot);
+        if (it != pos.end()) D.bnd_ci[e] = it->second;
+        else D.bnd_slot[e] = slot;
+      }
+    }
+  }
   // transfers
   for (int l = 0; l < out.nlev; ++l) {
     LevelDesc& D = out.lv[l];

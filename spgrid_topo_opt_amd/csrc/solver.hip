@@ -3,6 +3,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -107,11 +108,30 @@ void Solver::upload_layout() {
     upload(R.base, D.base);
     if (!D.restrict_src.empty()) upload(R.rsrc, D.restrict_src);
     if (!D.prolong_src.empty()) upload(R.psrc, D.prolong_src);
+    std::vector<int32_t> bnd;
+    R.d.bnd_max = 0;
     for (int c = 0; c < 8; ++c) {
-      if (!D.bnd_list[c].empty()) upload(R.bnd[c], D.bnd_list[c]);
-      R.d.bnd[c] = R.bnd[c].as<int>();
-      R.d.nbnd[c] = (int)D.bnd_list[c].size();
+      R.d.bnd_off[c] = (int)bnd.size();
+      bnd.insert(bnd.end(), D.bnd_list[c].begin(), D.bnd_list[c].end());
+      R.d.bnd_max = std::max(R.d.bnd_max, (int)D.bnd_list[c].size());
     }
+    R.d.bnd_off[8] = (int)bnd.size();
+    if (!bnd.empty()) upload(R.bnd, bnd);
+    R.d.bnd = R.bnd.as<int>();
+    if (!bnd.empty()) {
+      upload(R.bci, D.bnd_ci);
+      upload(R.bslot, D.bnd_slot);
+      R.bA64.alloc(bnd.size() * 288 * 8);
+      R.bA32.alloc(bnd.size() * 288 * 4);
+      R.bt64.alloc(bnd.size() * 3 * 8);
+      R.bt32.alloc(bnd.size() * 3 * 4);
+    }
+    R.d.bci = R.bci.as<int>();
+    R.d.bslot = R.bslot.as<int>();
+    R.d.bA64 = R.bA64.as<double>();
+    R.d.bA32 = R.bA32.as<float>();
+    R.d.bt64 = R.bt64.as<double>();
+    R.d.bt32 = R.bt32.as<float>();
     R.d.nbr = R.nbr.as<int>();
     R.d.flags = R.flags.as<uint8_t>();
     R.d.base = R.base.as<int>();
@@ -174,6 +194,7 @@ void Solver::build_operators() {
     launch_stencil_to_float(st_, R.st64.as<double>(), R.st32.as<float>(), (int64_t)D.nb * 243 * 64);
     sync();
   }
+  build_boundary_rows();
   // bottom: FP64 Cholesky of the Dirichlet-deleted stencil matrix, then its inverse
   // (Multigrid_Hierarchy.h:332-341; Elasticity_Matrix.cpp:15-233 factorizes with PARDISO)
   const int B = L - 1;
@@ -190,13 +211,12 @@ void Solver::build_operators() {
     throw SpgError(SPG_E_INVALID, "bottom level has " + std::to_string(nv) + " unknowns (limit " +
                                       std::to_string(kMaxBottomDofs) + "); increase mg_level");
   upload(var_of_, var);
-  Ainv_.alloc((size_t)nv * nv * 8);
-  xb_.alloc(nv * 8);
-  yb_.alloc(nv * 8);
+  DBuf Ainv;
+  Ainv.alloc((size_t)nv * nv * 8);
   info_.alloc(16);
-  HIPCHK(hipMemsetAsync(Ainv_.p, 0, (size_t)nv * nv * 8, st_));
-  launch_dense_assemble(st_, lv_[B].d, lv_[B].st64.as<double>(), var_of_.as<int>(), nv, Ainv_.as<double>());
-  RBCHK(rocsolver_dpotrf(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv_.as<double>(), (rocblas_int)nv,
+  HIPCHK(hipMemsetAsync(Ainv.p, 0, (size_t)nv * nv * 8, st_));
+  launch_dense_assemble(st_, lv_[B].d, lv_[B].st64.as<double>(), var_of_.as<int>(), nv, Ainv.as<double>());
+  RBCHK(rocsolver_dpotrf(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
                          info_.as<rocblas_int>()));
   int hinfo = 0;
   HIPCHK(hipMemcpyAsync(&hinfo, info_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
@@ -204,9 +224,37 @@ void Solver::build_operators() {
   if (hinfo != 0)
     throw SpgError(SPG_E_SINGULAR, "bottom Cholesky failed: leading minor " + std::to_string(hinfo) +
                                        " not positive definite");
-  RBCHK(rocsolver_dpotri(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv_.as<double>(), (rocblas_int)nv,
+  RBCHK(rocsolver_dpotri(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
                          info_.as<rocblas_int>()));
-  launch_symmetrize_lower(st_, Ainv_.as<double>(), nv);
+  launch_symmetrize_lower(st_, Ainv.as<double>(), nv);
+  // packed lower-triangle tiles (FP64 for the FP64 cycle, rounded to FP32 for the FP32 cycle)
+  nT_ = (int)((nv + 63) / 64);
+  const size_t ntile = (size_t)nT_ * (nT_ + 1) / 2;
+  const size_t npad = (size_t)nT_ * 64;
+  tiles64_.alloc(ntile * 4096 * 8);
+  tiles32_.alloc(ntile * 4096 * 4);
+  launch_pack_sym_tiles<double>(st_, Ainv.as<double>(), nv, nT_, tiles64_.as<double>());
+  launch_pack_sym_tiles<float>(st_, Ainv.as<double>(), nv, nT_, tiles32_.as<float>());
+  for (DBuf* d : {&xb64_, &yb64_}) d->alloc(npad * 8);
+  for (DBuf* d : {&xb32_, &yb32_}) d->alloc(npad * 4);
+  pb64_.alloc((size_t)symv_splits(nT_) * npad * 8);
+  pb32_.alloc((size_t)symv_splits(nT_) * npad * 4);
+  HIPCHK(hipMemsetAsync(xb64_.p, 0, npad * 8, st_));
+  HIPCHK(hipMemsetAsync(xb32_.p, 0, npad * 4, st_));
+  sync();
+}
+
+// coefficient rows of the boundary system of every smoothed level
+void Solver::build_boundary_rows() {
+  for (int l = 0; l < lay_.nlev; ++l) {
+    LevelRT& R = lv_[l];
+    const int n = R.d.bnd_off[8];
+    if (n == 0 || !R.bA64.p) continue;
+    if (l == 0) launch_bnd_coef0(st_, R.d, E64_.as<double>(), cfg_.dx, R.bA64.as<double>());
+    else if (lay_.lv[l].is_explicit) launch_bnd_coef_st(st_, R.d, R.st64.as<double>(), R.bA64.as<double>());
+    else continue;
+    launch_cast_d2f(st_, R.bA64.as<double>(), R.bA32.as<float>(), (int64_t)n * 288);
+  }
   sync();
 }
 
@@ -272,23 +320,19 @@ double Solver::read_scalar(const double* dev) {
 // ------------------------------------------------------------------ V-cycle
 template <class T>
 void Solver::smooth(int l, T* u, const T* b, bool forward) {
-  static const int order_st[2][8] = {{1, 5, 4, 6, 7, 3, 2, 0}, {0, 2, 3, 7, 6, 4, 5, 1}};
   const LevelRT& R = lv_[l];
-  auto sweep = [&](bool boundary) {
-    for (int c = 0; c < 8; ++c) {
-      if (l == 0) {
-        const int colour = forward ? c : 7 - c;
-        launch_gs0<T>(st_, R.d, E<T>(), dxT<T>(), u, b, colour, boundary);
-      } else {
-        const int colour = order_st[forward ? 0 : 1][c];
-        launch_st_gs<T>(st_, R.d, R.st<T>(), u, b, colour, boundary);
-      }
-    }
+  // Multigrid_Hierarchy.h:577-586 / 605-617: boundary sweeps, full sweeps, boundary sweeps
+  auto boundary = [&] {
+    if (l == 0) launch_gs0_boundary<T>(st_, R.d, E<T>(), dxT<T>(), u, b, cfg_.boundary_iters, forward);
+    else launch_st_boundary<T>(st_, R.d, R.st<T>(), u, b, cfg_.boundary_iters, forward);
   };
-  // Multigrid_Hierarchy.h:577-586 / 605-617
-  for (int i = 0; i < cfg_.boundary_iters; ++i) sweep(true);
-  for (int i = 0; i < cfg_.smooth_iters; ++i) sweep(false);
-  for (int i = 0; i < cfg_.boundary_iters; ++i) sweep(true);
+  boundary();
+  for (int i = 0; i < cfg_.smooth_iters; ++i)
+    for (int c = 0; c < 8; ++c) {
+      if (l == 0) launch_gs0<T>(st_, R.d, E<T>(), dxT<T>(), u, b, sweep_colour(true, forward, c));
+      else launch_st_gs<T>(st_, R.d, R.st<T>(), u, b, sweep_colour(false, forward, c));
+    }
+  boundary();
 }
 
 template <class T>
@@ -300,11 +344,12 @@ void Solver::level_residual(int l, const T* u, const T* b, T* r) {
 template <class T>
 void Solver::bottom(const T* b, T* u) {
   const LevelRT& R = lv_[lay_.nlev - 1];
-  const double one = 1.0, zero = 0.0;
-  launch_bottom_gather<T>(st_, R.d, var_of_.as<int>(), b, xb_.as<double>());
-  RBCHK(rocblas_dgemv(rb_, rocblas_operation_none, (rocblas_int)nvar_, (rocblas_int)nvar_, &one, Ainv_.as<double>(),
-                      (rocblas_int)nvar_, xb_.as<double>(), 1, &zero, yb_.as<double>(), 1));
-  launch_bottom_scatter<T>(st_, R.d, var_of_.as<int>(), yb_.as<double>(), b, u);
+  const bool f = sizeof(T) == 4;
+  T* x = f ? (T*)xb32_.p : (T*)xb64_.p;
+  T* y = f ? (T*)yb32_.p : (T*)yb64_.p;
+  launch_bottom_gather<T>(st_, R.d, var_of_.as<int>(), b, x);
+  launch_symv<T>(st_, nT_, f ? (const T*)tiles32_.p : (const T*)tiles64_.p, x, f ? (T*)pb32_.p : (T*)pb64_.p, y);
+  launch_bottom_scatter<T>(st_, R.d, var_of_.as<int>(), y, b, u);
 }
 
 template <class T>
@@ -607,14 +652,17 @@ void Solver::run_kernel(int op, int level, int precision, int forward, const dou
         if (level != 0 && !lay_.lv[level].is_explicit) throw SpgError(SPG_E_INVALID, "level has no smoother");
         put(A, in, nb);
         put(Bv, b, nb);
-        static const int order_st[2][8] = {{1, 5, 4, 6, 7, 3, 2, 0}, {0, 2, 3, 7, 6, 4, 5, 1}};
-        for (int c = 0; c < 8; ++c) {
-          if (level == 0)
-            launch_gs0<T>(st_, lv_[0].d, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), forward ? c : 7 - c,
-                          op == SPG_OP_GS_BOUNDARY);
-          else
-            launch_st_gs<T>(st_, lv_[level].d, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(),
-                            order_st[forward ? 0 : 1][c], op == SPG_OP_GS_BOUNDARY);
+        const LevelDev& Ld = lv_[level].d;
+        if (op == SPG_OP_GS_BOUNDARY) {
+          if (level == 0) launch_gs0_boundary<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), 1, forward != 0);
+          else launch_st_boundary<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), 1, forward != 0);
+        } else {
+          for (int c = 0; c < 8; ++c) {
+            if (level == 0)
+              launch_gs0<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(true, forward != 0, c));
+            else
+              launch_st_gs<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(false, forward != 0, c));
+          }
         }
         get(A.as<T>(), nb);
         break;
@@ -677,7 +725,7 @@ void Solver::level_stencils(int level, double* out) {
   HIPCHK(hipMemcpy(v.data(), lv_[level].st64.p, v.size() * 8, hipMemcpyDeviceToHost));
   for (int64_t b = 0; b < nb; ++b)
     for (int e = 0; e < 243; ++e)
-      for (int l = 0; l < 64; ++l) out[(b * 64 + l) * 243 + e] = v[(b * 243 + e) * 64 + l];
+      for (int l = 0; l < 64; ++l) out[(b * 64 + l) * 243 + e] = v[st_index(b, l, e)];
 }
 
 double Solver::bench_apply(int precision, int reps) {

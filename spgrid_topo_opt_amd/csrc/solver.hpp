@@ -46,8 +46,9 @@ struct DBuf {
 
 struct LevelRT {
   LevelDev d;
-  DBuf nbr, flags, rsrc, psrc, base, bnd[8];
+  DBuf nbr, flags, rsrc, psrc, base, bnd;
   DBuf st64, st32, cell_row, cfb, cgi;
+  DBuf bci, bslot, bA64, bA32, bt64, bt32;  // boundary system
   DBuf u32, b32, r32, u64, b64, r64;
   int64_t n3 = 0;  // vector length nb*192
   template <class T> T* u() const;
@@ -90,6 +91,7 @@ class Solver {
   template <class T> T dxT() const { return (T)cfg_.dx; }
   int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool with_ud);
   void build_operators();
+  void build_boundary_rows();
   void upload_layout();
   double read_scalar(const double* dev);
   void sync();
@@ -105,7 +107,8 @@ class Solver {
   int64_t n_ud_ = 0, n_f_ = 0;
   // bottom
   int64_t nvar_ = 0;
-  DBuf var_of_, Ainv_, xb_, yb_, info_;
+  int nT_ = 0;  // 64-row tiles of the bottom inverse
+  DBuf var_of_, info_, tiles64_, tiles32_, xb64_, yb64_, xb32_, yb32_, pb64_, pb32_;
   // PCG / outer vectors (level 0)
   DBuf x32_, rr32_, s32_, q32_, z32_, x64_, rr64_, s64_, q64_, z64_;
   DBuf u_, f_, res_, tmp_;

@@ -111,6 +111,25 @@ def test_gs_sweep_level0(blocky, prec, tol, forward, boundary):
     assert rel(got, want) < tol
 
 
+@pytest.mark.parametrize("path", ["1", "2"])
+def test_boundary_fallback_paths(blocky, path, monkeypatch):
+    """the global-memory boundary smoothers used when the boundary set does
+    not fit in LDS: same result as the oracle on every smoothed level"""
+    P = blocky
+    monkeypatch.setenv("SPG_BOUNDARY_PATH", path)
+    rng = np.random.default_rng(11)
+    for lv in [0] + list(range(2, P.o.levels)):
+        n = len(P.map[lv][0])
+        gf = P.map[lv][1]
+        u = rng.standard_normal((n, 3)) * 1e-3
+        u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+        b = rng.standard_normal((n, 3))
+        for prec, tol in ((1, 1e-11), (0, 1e-4)):
+            got = P.g.kernel(_lib.OP_GS_BOUNDARY, lv, u, b, precision=prec, forward=True)
+            want = P.from_o(lv, P.o.gs_sweep(lv, P.to_o(lv, u), P.to_o(lv, b), True, True))
+            assert rel(got, want) < tol, (lv, prec)
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 1e-6)])
 def test_restriction_and_prolongation(blocky, prec, tol):
     P = blocky
@@ -155,9 +174,11 @@ def test_stencil_gs_and_residual(blocky, prec, tol):
         want = P.from_o(lv, P.o.stencil_residual(lv, P.to_o(lv, u), P.to_o(lv, b)))
         assert rel(got, want) < tol
         for fwd in (True, False):
-            got = P.g.kernel(_lib.OP_GS_SWEEP, lv, u, b, precision=prec, forward=fwd)
-            want = P.from_o(lv, P.o.gs_sweep(lv, P.to_o(lv, u), P.to_o(lv, b), fwd, False))
-            assert rel(got, want) < tol * 10
+            for bnd in (False, True):
+                op = _lib.OP_GS_BOUNDARY if bnd else _lib.OP_GS_SWEEP
+                got = P.g.kernel(op, lv, u, b, precision=prec, forward=fwd)
+                want = P.from_o(lv, P.o.gs_sweep(lv, P.to_o(lv, u), P.to_o(lv, b), fwd, bnd))
+                assert rel(got, want) < tol * 10, (lv, fwd, bnd)
 
 
 def test_bottom_solve_is_exact(blocky):
