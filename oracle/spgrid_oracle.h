@@ -10,6 +10,14 @@
  *
  * Every function cites the reference file:line it restates.  Pinning of each
  * part (which tests check it against what) is listed in tests/test_oracle_*.py.
+ *
+ * Parity unpinned: the exact rounding trajectory of ORC_MODE_REF (where the
+ * reference rounds through float, and in which order its SIMD reductions
+ * add).  The reference cannot be built here, so that mode is only checked to
+ * converge to the pinned FP64 solution; every other function is pinned
+ * (element tables, apply, transfers, Galerkin stencils, Gauss-Seidel, bottom
+ * solve, PCG, sensitivity) against closed forms, scipy assemblies, dense
+ * textbook algorithms or finite differences.
  */
 #ifndef SPGRID_ORACLE_H
 #define SPGRID_ORACLE_H

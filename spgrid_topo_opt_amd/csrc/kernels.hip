@@ -621,8 +621,9 @@ __global__ void __launch_bounds__(256) k_st_list(const int* __restrict__ list, i
 // change during a boundary phase, so the phase runs on a compact system:
 // b~ = b - (row part of the nodes outside the boundary set) is formed once,
 // then every colour step of every sweep updates the boundary u held in LDS.
-// Coefficient rows: A[i*288 + t*72 + m*8 + g] is entry m = v*3+w of the 3x3
-// block of spoke s = g + 8t of boundary node i (zero for s >= 27).
+// Coefficient rows: A[i*288 + g*36 + t*9 + m] is entry m = v*3+w of the 3x3
+// block of spoke s = g + 8t of boundary node i (zero for s >= 27), so lane g
+// of an octet reads its 36 coefficients as one contiguous run.
 
 // level 0: A_s = dx * sum over the cells holding the node (corner I) and its
 // neighbour (corner I + o) of E_cell K0[I][I+o] (the rows of the
@@ -650,7 +651,7 @@ __global__ void k_bnd_coef0(int n, const int* __restrict__ list, const int* __re
       for (int m = 0; m < 9; ++m) acc[m] += e * cK0d[kb + m];
     }
   }
-  for (int m = 0; m < 9; ++m) A[(size_t)i * 288 + t * 72 + m * 8 + g] = dx * acc[m];
+  for (int m = 0; m < 9; ++m) A[(size_t)i * 288 + g * 36 + t * 9 + m] = dx * acc[m];
 }
 // explicit levels: the Galerkin stencil rows
 __global__ void k_bnd_coef_st(int n, const int* __restrict__ list, const double* __restrict__ ST,
@@ -660,10 +661,11 @@ __global__ void k_bnd_coef_st(int n, const int* __restrict__ list, const double*
   const int i = gid >> 5, t = (gid >> 3) & 3, g = gid & 7, s = g + 8 * t;
   const int node = list[i];
   for (int m = 0; m < 9; ++m)
-    A[(size_t)i * 288 + t * 72 + m * 8 + g] = s < 27 ? ST[st_index(node >> 6, node & 63, s * 9 + m)] : 0.0;
+    A[(size_t)i * 288 + g * 36 + t * 9 + m] = s < 27 ? ST[st_index(node >> 6, node & 63, s * 9 + m)] : 0.0;
 }
 
-// b~_i = b_i - sum over spokes to nodes outside the boundary set
+// b~_i = b_i - sum over spokes to nodes outside the boundary set (bt is
+// padded to 4 entries per node)
 template <class T>
 __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ list, const int* __restrict__ bslot,
                                                  const T* __restrict__ A, const T* __restrict__ u,
@@ -673,14 +675,13 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    if (t == 3 && g >= 3) break;
-    const int sl = live ? bslot[i * 32 + t * 8 + g] : -1;
+    const int sl = live ? bslot[i * 32 + g * 4 + t] : -1;
     if (sl >= 0) {
       const T x = u[vidx(sl >> 6, 0, sl & 63)], y = u[vidx(sl >> 6, 1, sl & 63)], z = u[vidx(sl >> 6, 2, sl & 63)];
-      const T* M = A + (size_t)i * 288 + t * 72 + g;
-      a0 += M[0] * x + M[8] * y + M[16] * z;
-      a1 += M[24] * x + M[32] * y + M[40] * z;
-      a2 += M[48] * x + M[56] * y + M[64] * z;
+      const T* M = A + (size_t)i * 288 + g * 36 + t * 9;
+      a0 += M[0] * x + M[1] * y + M[2] * z;
+      a1 += M[3] * x + M[4] * y + M[5] * z;
+      a2 += M[6] * x + M[7] * y + M[8] * z;
     }
   }
   a0 = octet_sum(a0);
@@ -688,9 +689,10 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
   a2 = octet_sum(a2);
   if (g == 0 && live) {
     const int node = list[i], blk = node >> 6, l = node & 63;
-    bt[i * 3 + 0] = b[vidx(blk, 0, l)] - a0;
-    bt[i * 3 + 1] = b[vidx(blk, 1, l)] - a1;
-    bt[i * 3 + 2] = b[vidx(blk, 2, l)] - a2;
+    bt[i * 4 + 0] = b[vidx(blk, 0, l)] - a0;
+    bt[i * 4 + 1] = b[vidx(blk, 1, l)] - a1;
+    bt[i * 4 + 2] = b[vidx(blk, 2, l)] - a2;
+    bt[i * 4 + 3] = T(0);
   }
 }
 
@@ -700,32 +702,128 @@ struct BndOff {
 // largest dynamic LDS of one workgroup on gfx950
 static constexpr size_t kMaxLds = 163840;
 
-// Per octet and pass: the coefficient row (36 values per lane), the
-// neighbour indices of the lane's spokes and b~ of the node.  Loaded one pass
-// ahead so that global-memory latency overlaps the previous pass.
+// Per octet and pass: the 36 coefficients of the lane's spokes, their
+// neighbour indices and b~ of the node, loaded one pass ahead (two register
+// sets used in turn) so global-memory latency overlaps the previous pass.
+// Loads are unconditional: octets past the end of a colour read node 0 and
+// discard the result, and spokes s >= 27 hold zero coefficients and index -1.
 template <class T>
 struct BndFetch {
-  T m[4][9];
+  T m[36];
   int j[4];
-  T r[3];
+  T r[4];
   int i;
   bool live;
 };
 template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double2;  // 16 bytes
+};
+template <class T>
 __device__ __forceinline__ void bnd_fetch(BndFetch<T>& F, int i, bool live, int g, const int* __restrict__ bci,
                                           const T* __restrict__ A, const T* __restrict__ bt) {
+  using V = typename Vec4<T>::type;
+  constexpr int PER = 16 / sizeof(T);
   F.i = i;
   F.live = live;
-  const T* M = A + (size_t)(live ? i : 0) * 288 + g;
+  const int k = live ? i : 0;
+  const V* M = reinterpret_cast<const V*>(A + (size_t)k * 288 + g * 36);
+#pragma unroll
+  for (int q = 0; q < 36 / PER; ++q) {
+    const V v = M[q];
+    if constexpr (PER == 4) {
+      F.m[4 * q] = v.x;
+      F.m[4 * q + 1] = v.y;
+      F.m[4 * q + 2] = v.z;
+      F.m[4 * q + 3] = v.w;
+    } else {
+      F.m[2 * q] = v.x;
+      F.m[2 * q + 1] = v.y;
+    }
+  }
+  const int4 jj = *reinterpret_cast<const int4*>(bci + (size_t)k * 32 + g * 4);
+  F.j[0] = live ? jj.x : -1;
+  F.j[1] = live ? jj.y : -1;
+  F.j[2] = live ? jj.z : -1;
+  F.j[3] = live ? jj.w : -1;
+  const V* R = reinterpret_cast<const V*>(bt + (size_t)k * 4);
+#pragma unroll
+  for (int q = 0; q < 4 / PER; ++q) {
+    const V v = R[q];
+    if constexpr (PER == 4) {
+      F.r[0] = v.x;
+      F.r[1] = v.y;
+      F.r[2] = v.z;
+      F.r[3] = v.w;
+    } else {
+      F.r[2 * q] = v.x;
+      F.r[2 * q + 1] = v.y;
+    }
+  }
+}
+// one pass: the row of node F.i over its boundary neighbours (LDS), then the
+// update of the node by lane g = 5, which holds spoke 13 (the self block)
+template <class T>
+__device__ __forceinline__ void bnd_pass(const BndFetch<T>& F, int g, int n, T* ub, const uint8_t* fb) {
+  T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const bool has = live && (t < 3 || g < 3);
-    F.j[t] = has ? bci[i * 32 + t * 8 + g] : -1;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) F.m[t][q] = has ? M[t * 72 + q * 8] : T(0);
+    const int j = F.j[t];
+    if (j >= 0) {
+      const T x = ub[j], y = ub[n + j], z = ub[2 * n + j];
+      a0 += F.m[t * 9 + 0] * x + F.m[t * 9 + 1] * y + F.m[t * 9 + 2] * z;
+      a1 += F.m[t * 9 + 3] * x + F.m[t * 9 + 4] * y + F.m[t * 9 + 5] * z;
+      a2 += F.m[t * 9 + 6] * x + F.m[t * 9 + 7] * y + F.m[t * 9 + 8] * z;
+    }
   }
+  a0 = octet_sum(a0);
+  a1 = octet_sum(a1);
+  a2 = octet_sum(a2);
+  if (g == 5 && F.live) {
+    const int i = F.i;
+    const uint8_t fl = fb[i];
+    if (fl & 1) {
+      T d[6] = {F.m[9], F.m[13], F.m[17], F.m[10], F.m[11], F.m[14]};
+      dirichlet_inverse(d, fl);
+      const T r0 = F.r[0] - a0, r1 = F.r[1] - a1, r2 = F.r[2] - a2;
+      const T u0 = ub[i] + d[0] * r0 + d[3] * r1 + d[4] * r2;
+      const T u1 = ub[n + i] + d[3] * r0 + d[1] * r1 + d[5] * r2;
+      const T u2 = ub[2 * n + i] + d[4] * r0 + d[5] * r1 + d[2] * r2;
+      ub[i] = (fl & 2) ? T(0) : u0;
+      ub[n + i] = (fl & 4) ? T(0) : u1;
+      ub[2 * n + i] = (fl & 8) ? T(0) : u2;
+    } else {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) F.r[c] = (live && g == 5) ? bt[i * 3 + c] : T(0);
+      for (int cc = 0; cc < 3; ++cc)
+        if (fl & (2 << cc)) ub[cc * n + i] = T(0);
+    }
+  }
+}
+// position in the pass sequence (sweep, colour step, pass within the colour)
+struct BndCursor {
+  int sw, ci, p, c;
+};
+template <int NO>
+__device__ __forceinline__ BndCursor bnd_advance(const BndCursor& a, const BndOff& L, const ColourOrder& ord,
+                                                 bool& step_end) {
+  BndCursor b = a;
+  b.p = a.p + 1;
+  step_end = L.off[a.c] + b.p * NO >= L.off[a.c + 1];
+  if (step_end) {
+    b.p = 0;
+    if (++b.ci == 8) {
+      b.ci = 0;
+      ++b.sw;
+    }
+    b.c = ord.at(b.ci);
+  }
+  return b;
 }
 
 template <class T, int NT>
@@ -744,72 +842,30 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
     fb[i] = flags[node];
   }
   const int oct = threadIdx.x >> 3, g = threadIdx.x & 7;
-  int sw = 0, ci = 0, p = 0, c = ord.at(0);
-  BndFetch<T> cur, nxt;
-  {
-    const int i = L.off[c] + oct;
-    bnd_fetch<T>(cur, i, i < L.off[c + 1], g, bci, A, bt);
-  }
+  auto fetch = [&](BndFetch<T>& F, const BndCursor& k) {
+    const int i = L.off[k.c] + k.p * NO + oct;
+    bnd_fetch<T>(F, i, i < L.off[k.c + 1], g, bci, A, bt);
+  };
+  BndCursor cur{0, 0, 0, ord.at(0)};
+  BndFetch<T> X, Y;
+  fetch(X, cur);
   __syncthreads();
   while (true) {
-    // the pass after this one
-    int sw2 = sw, ci2 = ci, p2 = p + 1, c2 = c;
-    const bool step_end = L.off[c] + p2 * NO >= L.off[c + 1];
-    if (step_end) {
-      p2 = 0;
-      if (++ci2 == 8) {
-        ci2 = 0;
-        ++sw2;
-      }
-      c2 = ord.at(ci2);
-    }
-    const bool more = sw2 < sweeps;
-    if (more) {
-      const int i2 = L.off[c2] + p2 * NO + oct;
-      bnd_fetch<T>(nxt, i2, i2 < L.off[c2 + 1], g, bci, A, bt);
-    }
-    // this pass: row of node cur.i over the boundary neighbours (LDS)
-    T a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int j = cur.j[t];
-      if (j >= 0) {
-        const T x = ub[j], y = ub[n + j], z = ub[2 * n + j];
-        a0 += cur.m[t][0] * x + cur.m[t][1] * y + cur.m[t][2] * z;
-        a1 += cur.m[t][3] * x + cur.m[t][4] * y + cur.m[t][5] * z;
-        a2 += cur.m[t][6] * x + cur.m[t][7] * y + cur.m[t][8] * z;
-      }
-    }
-    a0 = octet_sum(a0);
-    a1 = octet_sum(a1);
-    a2 = octet_sum(a2);
-    if (g == 5 && cur.live) {
-      const int i = cur.i;
-      const uint8_t fl = fb[i];
-      if (fl & 1) {
-        // spoke 13 (the node itself) is group t = 1 of lane g = 5
-        T d[6] = {cur.m[1][0], cur.m[1][4], cur.m[1][8], cur.m[1][1], cur.m[1][2], cur.m[1][5]};
-        dirichlet_inverse(d, fl);
-        const T r0 = cur.r[0] - a0, r1 = cur.r[1] - a1, r2 = cur.r[2] - a2;
-        const T u0 = ub[i] + d[0] * r0 + d[3] * r1 + d[4] * r2;
-        const T u1 = ub[n + i] + d[3] * r0 + d[1] * r1 + d[5] * r2;
-        const T u2 = ub[2 * n + i] + d[4] * r0 + d[5] * r1 + d[2] * r2;
-        ub[i] = (fl & 2) ? T(0) : u0;
-        ub[n + i] = (fl & 4) ? T(0) : u1;
-        ub[2 * n + i] = (fl & 8) ? T(0) : u2;
-      } else {
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-          if (fl & (2 << cc)) ub[cc * n + i] = T(0);
-      }
-    }
-    if (step_end) __syncthreads();
+    bool end;
+    BndCursor nx = bnd_advance<NO>(cur, L, ord, end);
+    bool more = nx.sw < sweeps;
+    fetch(Y, more ? nx : cur);
+    bnd_pass<T>(X, g, n, ub, fb);
+    if (end) __syncthreads();
     if (!more) break;
-    cur = nxt;
-    sw = sw2;
-    ci = ci2;
-    p = p2;
-    c = c2;
+    cur = nx;
+    nx = bnd_advance<NO>(cur, L, ord, end);
+    more = nx.sw < sweeps;
+    fetch(X, more ? nx : cur);
+    bnd_pass<T>(Y, g, n, ub, fb);
+    if (end) __syncthreads();
+    if (!more) break;
+    cur = nx;
   }
   for (int i = threadIdx.x; i < n; i += NT) {
     const int node = list[i], blk = node >> 6, l = node & 63;

@@ -23,7 +23,7 @@ struct LevelDev {
   int bnd_off[9] = {};             // colour c owns bnd[bnd_off[c] .. bnd_off[c+1])
   int bnd_max = 0;                 // longest colour list
   // boundary system (LDS-resident boundary phase): neighbour tables [n*32]
-  // and coefficient rows [n*288] per precision, plus the b~ scratch [n*3]
+  // and coefficient rows [n*288] per precision, plus the b~ scratch [n*4]
   const int* bci = nullptr;
   const int* bslot = nullptr;
   const float* bA32 = nullptr;

@@ -365,7 +365,7 @@ Layout build_layout(const ProblemDesc& p) {
         const int32_t nb = D.nbr[(size_t)g * 27 + rs];
         if (nb < 0) continue;
         const int32_t slot = nb * 64 + (((a & 3) << 4) | ((b & 3) << 2) | (c & 3));
-        const size_t e = i * 32 + (size_t)(sp >> 3) * 8 + (sp & 7);
+        const size_t e = i * 32 + (size_t)(sp & 7) * 4 + (sp >> 3);
         auto it = pos.find(slot);
         if (it != pos.end()) D.bnd_ci[e] = it->second;
         else D.bnd_slot[e] = slot;

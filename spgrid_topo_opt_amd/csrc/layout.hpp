@@ -97,7 +97,7 @@ struct LevelDesc {
   std::vector<int32_t> bnd_list[8];   // boundary nodes per colour: blk*64+l
   std::vector<int32_t> dir_list;      // nodes with any Dirichlet bit: blk*64+l
   // boundary system: for boundary node i (list order, colours concatenated)
-  // and spoke s = g + 8t, entry i*32 + t*8 + g holds the neighbour's index in
+  // and spoke s = g + 8t, entry i*32 + g*4 + t holds the neighbour's index in
   // the boundary list (bnd_ci) or, for a neighbour outside it, its slot
   // blk*64+l (bnd_slot); -1 where absent or not applicable.
   std::vector<int32_t> bnd_ci, bnd_slot;

@@ -123,8 +123,8 @@ void Solver::upload_layout() {
       upload(R.bslot, D.bnd_slot);
       R.bA64.alloc(bnd.size() * 288 * 8);
       R.bA32.alloc(bnd.size() * 288 * 4);
-      R.bt64.alloc(bnd.size() * 3 * 8);
-      R.bt32.alloc(bnd.size() * 3 * 4);
+      R.bt64.alloc(bnd.size() * 4 * 8);
+      R.bt32.alloc(bnd.size() * 4 * 4);
     }
     R.d.bci = R.bci.as<int>();
     R.d.bslot = R.bslot.as<int>();
