@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the FP32 apply and a 512 MiB copy (5 launches each) for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -57,6 +59,10 @@ def main():
     s = _lib.Solver(prob)
     setup_s = time.perf_counter() - t0
     log(f"[bench] solver setup {setup_s:.2f}s levels={s.levels}")
+
+    if args.traffic_probe:
+        traffic_probe(s, prob)
+        return
 
     def mixed():
         return s.solve_mixed(tolerance=args.tol, want_output=False)[1]
@@ -141,6 +147,25 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+COPY_BYTES = 512 << 20
+
+
+def traffic_probe(s, prob):
+    """Five launches of the level-0 FP32 apply and five of a plain device copy
+    of COPY_BYTES (read COPY_BYTES + write COPY_BYTES).  Under
+    `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) the copy's per-dispatch
+    counter gives the KiB->bytes scale applied to the apply's counter."""
+    import torch
+
+    s.bench_apply(0, 5)
+    x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"[bench] traffic probe done: FP32 apply x5, copy of {COPY_BYTES} B x5")
 
 
 def cpu_baseline(prob, gpu_iterations, budget_s):

@@ -71,7 +71,9 @@ typedef struct {
 /* Solve parameters (krylov + defect-correction knobs of FEMInputs). */
 typedef struct {
   double tolerance;        /* target ||P(f-Ku)||_inf <= tolerance*||r0||_inf */
-  double inner_tolerance;  /* per-pass inner CG reduction (<=0: tolerance)  */
+  double inner_tolerance;  /* per-pass inner CG reduction; <=0: each pass aims
+                              at the target (an FP32 pass at most 1e-6 of its
+                              starting residual)                             */
   int32_t max_iterations;  /* PCG iterations per pass                       */
   int32_t max_passes;      /* FP64 defect-correction passes (>=1)           */
   int32_t precision;       /* 0: FP32 inner / FP64 outer (mixed), 1: FP64   */
@@ -160,6 +162,12 @@ int spg_level_stencils(spg_solver *s, int32_t level, double *out);
 /* Time `reps` launches of the level-0 stiffness apply on device-resident
  * data; returns the average kernel time in seconds (HIP events).            */
 int spg_bench_apply(spg_solver *s, int32_t precision, int32_t reps, double *seconds);
+/* Time `reps` runs of one operator (SPG_OP_*) of `level` on device-resident
+ * vectors (HIP events on the solver's stream); returns seconds per run.
+ * SPG_OP_GS_SWEEP times one full sweep (8 colour steps), SPG_OP_GS_BOUNDARY
+ * one boundary phase of `arg` sweeps, SPG_OP_VCYCLE one V-cycle (level 0). */
+int spg_bench_kernel(spg_solver *s, int32_t op, int32_t level, int32_t precision, int32_t arg, int32_t reps,
+                     double *seconds);
 
 #ifdef __cplusplus
 }

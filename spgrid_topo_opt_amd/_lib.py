@@ -30,7 +30,7 @@ EXPORTS = [
     "fem_solve", "fem_solve_test", "spg_last_error", "spg_version", "spg_device_count", "spg_create",
     "spg_destroy", "spg_update_density", "spg_set_forces", "spg_solve", "spg_get_displacement_device",
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
-    "spg_kernel", "spg_level_stencils", "spg_bench_apply",
+    "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel",
 ]
 
 
@@ -127,6 +127,7 @@ def lib():
         L.spg_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp]
         L.spg_level_stencils.argtypes = [vp, C.c_int32, dp]
         L.spg_bench_apply.argtypes = [vp, C.c_int32, C.c_int32, dp]
+        L.spg_bench_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -189,8 +190,10 @@ class Solver:
         self.set_forces(prob.force_coord, prob.force_value)
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().spg_destroy(self.h)
+        # at interpreter shutdown the module globals may already be gone
+        L = globals().get("_lib")
+        if getattr(self, "h", None) and L is not None:
+            L.spg_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -218,8 +221,10 @@ class Solver:
         return out, d
 
     def solve_mixed(self, tolerance=1e-6, **kw):
-        """FP32 inner MG-PCG with FP64 defect correction."""
-        kw.setdefault("inner_tolerance", 1e-3)
+        """FP32 inner MG-PCG with FP64 defect correction.  With inner_tolerance 0
+        each pass aims at the target itself; a further pass runs only if the
+        FP64 residual after a pass still misses it."""
+        kw.setdefault("inner_tolerance", 0.0)
         kw.setdefault("max_passes", 30)
         return self.solve(tolerance=tolerance, precision=0, **kw)
 
@@ -227,7 +232,7 @@ class Solver:
     def displacement_device(self, ptr):
         _check(lib().spg_get_displacement_device(self.h, C.c_void_p(ptr)))
 
-    def solve_adjoint_device(self, g_ptr, lam_ptr, tolerance=1e-6, inner_tolerance=1e-3, max_iterations=500,
+    def solve_adjoint_device(self, g_ptr, lam_ptr, tolerance=1e-6, inner_tolerance=0.0, max_iterations=500,
                              max_passes=30, precision=0):
         p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, 0)
         info = Info()
@@ -278,6 +283,12 @@ class Solver:
     def bench_apply(self, precision=0, reps=20):
         s = C.c_double(0.0)
         _check(lib().spg_bench_apply(self.h, precision, reps, C.byref(s)))
+        return s.value
+
+    def bench_kernel(self, op, level=0, precision=0, arg=0, reps=10):
+        """seconds per run of one operator on device-resident vectors"""
+        s = C.c_double(0.0)
+        _check(lib().spg_bench_kernel(self.h, op, level, precision, arg, reps, C.byref(s)))
         return s.value
 
 

@@ -226,6 +226,15 @@ int spg_bench_apply(spg_solver* s, int32_t precision, int32_t reps, double* seco
   API_CATCH
 }
 
+int spg_bench_kernel(spg_solver* s, int32_t op, int32_t level, int32_t precision, int32_t arg, int32_t reps,
+                     double* seconds) {
+  API_TRY
+  if (!s || !seconds || reps <= 0) throw SpgError(SPG_E_INVALID, "bad argument");
+  *seconds = s->s->bench_kernel(op, level, precision, arg, reps);
+  return SPG_OK;
+  API_CATCH
+}
+
 // ------------------------------------------------------------------ wire
 static void emit(const FemOutputs& o, void* output) {
   WireWriter w;
@@ -322,7 +331,7 @@ void fem_solve(void* input, void* output) {
     const int dci = in.get_int("defect_correction_iter", 0);
     prm.max_passes = dci > 0 ? dci : 1;
     prm.precision = in.get_int("fp64_solve", 0) ? 1 : 0;
-    prm.inner_tolerance = in.get_double("inner_tolerance", prm.max_passes > 1 ? 1e-3 : 0.0);
+    prm.inner_tolerance = in.get_double("inner_tolerance", 0.0);
     spg_info info{};
     const int rc = h->s->solve(prm, &info);
     out.displacements.blocks.resize(nb);

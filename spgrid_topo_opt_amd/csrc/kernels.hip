@@ -399,6 +399,92 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_gs0_colour(int nb, int colour
   }
 }
 
+// One colour of the level-0 sweep with a corner per wavefront: a 512-thread
+// workgroup owns the 64 colour-c nodes of 8 consecutive blocks and stages
+// the 5^3 nodes and 4^3 cells around each block's colour nodes once; wave I
+// adds the rows of cell (node - I) for all 64 nodes, so K0[I][J] is
+// wave-uniform (scalar registers).  The 8 partial rows meet in LDS and wave 0
+// updates the nodes (SPGrid_Gauss_Seidel.h:765-875).  Races: as for
+// k_gs0_colour, a halo never holds a colour-c node of another block.
+static constexpr int UC_BLOCKS = 8;
+template <class T>
+__global__ void __launch_bounds__(512) k_gs0_colour_uc(int nb, int colour, const int* __restrict__ nbr,
+                                                       const uint8_t* __restrict__ flags, const T* __restrict__ E,
+                                                       T dx, T* __restrict__ u, const T* __restrict__ b) {
+  __shared__ T hu[3][UC_BLOCKS * 125];
+  __shared__ T he[UC_BLOCKS * 64];
+  __shared__ T pf[8][3][64];
+  __shared__ int ht[UC_BLOCKS][27];
+  const int tid = threadIdx.x;
+  const int blk0 = xcd_chunk(blockIdx.x, gridDim.x) * UC_BLOCKS;
+  const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
+  if (tid < UC_BLOCKS * 27) {
+    const int bl = tid / 27, sp = tid - bl * 27, blk = blk0 + bl;
+    ht[bl][sp] = blk < nb ? (sp == 13 ? blk : nbr[blk * 27 + sp]) : -1;
+  }
+  __syncthreads();
+  for (int h = tid; h < UC_BLOCKS * 125; h += 512) {
+    const int bl = h / 125, hh = h - bl * 125;
+    const int x = px - 1 + hh / 25, y = py - 1 + (hh / 5) % 5, z = pz - 1 + hh % 5;
+    const int bb = ht[bl][ring_spoke(x, y, z)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hu[c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+  }
+  {
+    const int bl = tid >> 6, hh = tid & 63;
+    const int x = px - 1 + (hh >> 4), y = py - 1 + ((hh >> 2) & 3), z = pz - 1 + (hh & 3);
+    const int bb = ht[bl][ring_spoke(x, y, z)];
+    he[tid] = bb >= 0 ? E[bb * 64 + loc3(x, y, z)] : T(0);
+  }
+  __syncthreads();
+  const int lane = tid & 63;
+  const int I = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int bl = lane >> 3, k = lane & 7;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;  // halo coords
+  {
+    const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
+    const int cx = nx - ix, cy = ny - iy, cz = nz - iz;  // cell min corner (halo coords)
+    const T e = he[bl * 64 + (cx * 4 + cy) * 4 + cz];
+    T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int J = 0; J < 8; ++J) {
+      const int h = bl * 125 + ((cx + ((J >> 2) & 1)) * 5 + (cy + ((J >> 1) & 1))) * 5 + (cz + (J & 1));
+      const T v0 = hu[0][h], v1 = hu[1][h], v2 = hu[2][h];
+      const int kb = (I * 8 + J) * 9;
+      t0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
+      t1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
+      t2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
+    }
+    pf[I][0][lane] = e * t0;
+    pf[I][1][lane] = e * t1;
+    pf[I][2][lane] = e * t2;
+  }
+  __syncthreads();
+  if (I != 0) return;
+  const int blk = blk0 + bl;
+  if (blk >= nb) return;
+  T f0 = 0, f1 = 0, f2 = 0, d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    f0 += pf[c][0][lane];
+    f1 += pf[c][1][lane];
+    f2 += pf[c][2][lane];
+    const T e = he[bl * 64 + ((nx - ((c >> 2) & 1)) * 4 + (ny - ((c >> 1) & 1))) * 4 + (nz - (c & 1))];
+    const int kd = (c * 8 + c) * 9;
+    d0 += e * KT<T>::k(kd + 0);
+    d1 += e * KT<T>::k(kd + 4);
+    d2 += e * KT<T>::k(kd + 8);
+    d3 += e * KT<T>::k(kd + 1);
+    d4 += e * KT<T>::k(kd + 2);
+    d5 += e * KT<T>::k(kd + 5);
+  }
+  const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+  const int hn = bl * 125 + (nx * 5 + ny) * 5 + nz;
+  T d[6] = {dx * d0, dx * d1, dx * d2, dx * d3, dx * d4, dx * d5};
+  gs_node_update<T>(flags[blk * 64 + l], d, dx * f0, dx * f1, dx * f2, blk, l, hu[0][hn], hu[1][hn], hu[2][hn], b, u);
+}
+
 // Level-0 update of list node `s` (blk*64+l) by the 8 lanes of an octet,
 // reading straight from global memory (boundary lists).
 template <class T>
@@ -705,8 +791,7 @@ static constexpr size_t kMaxLds = 163840;
 // Per octet and pass: the 36 coefficients of the lane's spokes, their
 // neighbour indices and b~ of the node, loaded one pass ahead (two register
 // sets used in turn) so global-memory latency overlaps the previous pass.
-// Loads are unconditional: octets past the end of a colour read node 0 and
-// discard the result, and spokes s >= 27 hold zero coefficients and index -1.
+// Spokes s >= 27 hold zero coefficients and index -1.
 template <class T>
 struct BndFetch {
   T m[36];
@@ -732,8 +817,17 @@ __device__ __forceinline__ void bnd_fetch(BndFetch<T>& F, int i, bool live, int 
   constexpr int PER = 16 / sizeof(T);
   F.i = i;
   F.live = live;
-  const int k = live ? i : 0;
-  const V* M = reinterpret_cast<const V*>(A + (size_t)k * 288 + g * 36);
+  if (!live) {
+    // octets past the end of the colour load nothing
+#pragma unroll
+    for (int q = 0; q < 36; ++q) F.m[q] = T(0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) F.j[t] = -1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) F.r[q] = T(0);
+    return;
+  }
+  const V* M = reinterpret_cast<const V*>(A + (size_t)i * 288 + g * 36);
 #pragma unroll
   for (int q = 0; q < 36 / PER; ++q) {
     const V v = M[q];
@@ -747,24 +841,29 @@ __device__ __forceinline__ void bnd_fetch(BndFetch<T>& F, int i, bool live, int 
       F.m[2 * q + 1] = v.y;
     }
   }
-  const int4 jj = *reinterpret_cast<const int4*>(bci + (size_t)k * 32 + g * 4);
-  F.j[0] = live ? jj.x : -1;
-  F.j[1] = live ? jj.y : -1;
-  F.j[2] = live ? jj.z : -1;
-  F.j[3] = live ? jj.w : -1;
-  const V* R = reinterpret_cast<const V*>(bt + (size_t)k * 4);
+  const int4 jj = *reinterpret_cast<const int4*>(bci + (size_t)i * 32 + g * 4);
+  F.j[0] = jj.x;
+  F.j[1] = jj.y;
+  F.j[2] = jj.z;
+  F.j[3] = jj.w;
+  if (g == 5) {
+    const V* R = reinterpret_cast<const V*>(bt + (size_t)i * 4);
 #pragma unroll
-  for (int q = 0; q < 4 / PER; ++q) {
-    const V v = R[q];
-    if constexpr (PER == 4) {
-      F.r[0] = v.x;
-      F.r[1] = v.y;
-      F.r[2] = v.z;
-      F.r[3] = v.w;
-    } else {
-      F.r[2 * q] = v.x;
-      F.r[2 * q + 1] = v.y;
+    for (int q = 0; q < 4 / PER; ++q) {
+      const V v = R[q];
+      if constexpr (PER == 4) {
+        F.r[0] = v.x;
+        F.r[1] = v.y;
+        F.r[2] = v.z;
+        F.r[3] = v.w;
+      } else {
+        F.r[2 * q] = v.x;
+        F.r[2 * q + 1] = v.y;
+      }
     }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) F.r[q] = T(0);
   }
 }
 // one pass: the row of node F.i over its boundary neighbours (LDS), then the
@@ -880,15 +979,35 @@ struct BndThreads {
 };
 
 // ----------------------------------------------------------------- transfers
-// R = P^T, full weighting {1/2,1,1/2}^3 around 2c-B (Restriction.h:52-92)
+// Transfers, LDS-staged: a wavefront owns one block of the output level.
+// Restriction R = P^T is full weighting {1/2,1,1/2}^3 around the fine node
+// 2c-B (Restriction.h:52-92); the wave stages the 9^3 fine nodes
+// x = 2l + d (d = -1..1) of its coarse block (fine block (x>>2)+1 of the
+// 27-entry source table).  Prolongation adds the trilinear interpolant of
+// the 3^3 coarse nodes around the fine block, then applies the Dirichlet and
+// null-space projections (Prolongation.h:57-99, Multigrid_Hierarchy.h:600-603).
+static constexpr int TR_WAVES = 2;
 template <class T>
-__global__ void __launch_bounds__(TPB) k_restrict(int nbc, const int* __restrict__ rsrc,
-                                                  const uint8_t* __restrict__ cflags, const T* __restrict__ fine,
-                                                  T* __restrict__ coarse) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nbc * 64) return;
-  const int blk = gid >> 6, l = gid & 63;
-  const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
+__global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const int* __restrict__ rsrc,
+                                                                const uint8_t* __restrict__ cflags,
+                                                                const T* __restrict__ fine, T* __restrict__ coarse) {
+  __shared__ T hf[TR_WAVES][3][729];
+  __shared__ int ht[TR_WAVES][27];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_chunk(blockIdx.x, gridDim.x) * TR_WAVES + w;
+  const bool live = blk < nbc;
+  if (lane < 27) ht[w][lane] = live ? rsrc[blk * 27 + lane] : -1;
+  __syncthreads();
+  for (int h = lane; h < 729; h += 64) {
+    const int x = h / 81 - 1, y = (h / 9) % 9 - 1, z = h % 9 - 1;
+    const int fb = ht[w][((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((z >> 2) + 1)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hf[w][c][h] = fb >= 0 ? fine[vidx(fb, c, q)] : T(0);
+  }
+  __syncthreads();
+  if (!live) return;
+  const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int di = -1; di <= 1; ++di)
@@ -896,35 +1015,42 @@ __global__ void __launch_bounds__(TPB) k_restrict(int nbc, const int* __restrict
     for (int dj = -1; dj <= 1; ++dj)
 #pragma unroll
       for (int dk = -1; dk <= 1; ++dk) {
-        const int x = 2 * li + di, y = 2 * lj + dj, z = 2 * lk + dk;
-        const int fb = rsrc[blk * 27 + ((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((z >> 2) + 1)];
-        if (fb < 0) continue;
-        const T w = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5) * (dk == 0 ? 1.0 : 0.5));
-        const int q = loc3(x, y, z);
-        a0 += w * fine[vidx(fb, 0, q)];
-        a1 += w * fine[vidx(fb, 1, q)];
-        a2 += w * fine[vidx(fb, 2, q)];
+        const T wgt = T((di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5) * (dk == 0 ? 1.0 : 0.5));
+        const int h = (2 * li + di + 1) * 81 + (2 * lj + dj + 1) * 9 + (2 * lk + dk + 1);
+        a0 += wgt * hf[w][0][h];
+        a1 += wgt * hf[w][1][h];
+        a2 += wgt * hf[w][2][h];
       }
-  const uint8_t fl = cflags[gid];
-  coarse[vidx(blk, 0, l)] = (fl & 2) ? T(0) : a0;
-  coarse[vidx(blk, 1, l)] = (fl & 4) ? T(0) : a1;
-  coarse[vidx(blk, 2, l)] = (fl & 8) ? T(0) : a2;
+  const uint8_t fl = cflags[blk * 64 + lane];
+  coarse[vidx(blk, 0, lane)] = (fl & 2) ? T(0) : a0;
+  coarse[vidx(blk, 1, lane)] = (fl & 4) ? T(0) : a1;
+  coarse[vidx(blk, 2, lane)] = (fl & 8) ? T(0) : a2;
 }
 
-// fine += P coarse, then Dirichlet and null-space projection
-// (Prolongation.h:57-99, Multigrid_Hierarchy.h:600-603)
 template <class T>
-__global__ void __launch_bounds__(TPB) k_prolong(int nbf, const int* __restrict__ psrc, const int* __restrict__ fbase,
-                                                 const uint8_t* __restrict__ fflags, const T* __restrict__ coarse,
-                                                 T* __restrict__ fine) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nbf * 64) return;
-  const int blk = gid >> 6, l = gid & 63;
-  const int li = l >> 4, lj = (l >> 2) & 3, lk = l & 3;
-  const int phx = ((fbase[3 * blk] + 4) >> 1) & 3, phy = ((fbase[3 * blk + 1] + 4) >> 1) & 3,
-            phz = ((fbase[3 * blk + 2] + 8) >> 1) & 3;
-  const uint8_t fl = fflags[gid];
-  T a0 = fine[vidx(blk, 0, l)], a1 = fine[vidx(blk, 1, l)], a2 = fine[vidx(blk, 2, l)];
+__global__ void __launch_bounds__(64 * TR_WAVES) k_prolong_lds(int nbf, const int* __restrict__ psrc,
+                                                               const int* __restrict__ fbase,
+                                                               const uint8_t* __restrict__ fflags,
+                                                               const T* __restrict__ coarse, T* __restrict__ fine) {
+  __shared__ T hc[TR_WAVES][3][27];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_chunk(blockIdx.x, gridDim.x) * TR_WAVES + w;
+  const bool live = blk < nbf;
+  const int b0 = live ? blk : 0;
+  const int phx = ((fbase[3 * b0] + 4) >> 1) & 3, phy = ((fbase[3 * b0 + 1] + 4) >> 1) & 3,
+            phz = ((fbase[3 * b0 + 2] + 8) >> 1) & 3;
+  if (lane < 27) {
+    const int x = phx + lane / 9, y = phy + (lane / 3) % 3, z = phz + lane % 3;
+    const int cb = live ? psrc[blk * 8 + ((x >> 2) << 2) + ((y >> 2) << 1) + (z >> 2)] : -1;
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hc[w][c][lane] = cb >= 0 ? coarse[vidx(cb, c, q)] : T(0);
+  }
+  __syncthreads();
+  if (!live) return;
+  const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+  const uint8_t fl = fflags[blk * 64 + lane];
+  T a0 = fine[vidx(blk, 0, lane)], a1 = fine[vidx(blk, 1, lane)], a2 = fine[vidx(blk, 2, lane)];
 #pragma unroll
   for (int oi = 0; oi < 2; ++oi)
 #pragma unroll
@@ -934,20 +1060,16 @@ __global__ void __launch_bounds__(TPB) k_prolong(int nbf, const int* __restrict_
         const T wi = (li & 1) ? T(0.5) : (oi ? T(0) : T(1));
         const T wj = (lj & 1) ? T(0.5) : (oj ? T(0) : T(1));
         const T wk = (lk & 1) ? T(0.5) : (ok ? T(0) : T(1));
-        const T w = wi * wj * wk;
-        if (w == T(0)) continue;
-        const int x = phx + (li >> 1) + oi, y = phy + (lj >> 1) + oj, z = phz + (lk >> 1) + ok;
-        const int cb = psrc[blk * 8 + ((x >> 2) << 2) + ((y >> 2) << 1) + (z >> 2)];
-        if (cb < 0) continue;
-        const int q = loc3(x, y, z);
-        a0 += w * coarse[vidx(cb, 0, q)];
-        a1 += w * coarse[vidx(cb, 1, q)];
-        a2 += w * coarse[vidx(cb, 2, q)];
+        const T wgt = wi * wj * wk;
+        const int h = ((li >> 1) + oi) * 9 + ((lj >> 1) + oj) * 3 + ((lk >> 1) + ok);
+        a0 += wgt * hc[w][0][h];
+        a1 += wgt * hc[w][1][h];
+        a2 += wgt * hc[w][2][h];
       }
   const bool act = fl & 1;
-  fine[vidx(blk, 0, l)] = (!act || (fl & 2)) ? T(0) : a0;
-  fine[vidx(blk, 1, l)] = (!act || (fl & 4)) ? T(0) : a1;
-  fine[vidx(blk, 2, l)] = (!act || (fl & 8)) ? T(0) : a2;
+  fine[vidx(blk, 0, lane)] = (!act || (fl & 2)) ? T(0) : a0;
+  fine[vidx(blk, 1, lane)] = (!act || (fl & 4)) ? T(0) : a1;
+  fine[vidx(blk, 2, lane)] = (!act || (fl & 8)) ? T(0) : a2;
 }
 
 // ------------------------------------------------------------------- vectors
@@ -1426,10 +1548,17 @@ void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, dou
   if (n > 0) k_bnd_coef_st<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, ST, A);
 }
 
+// SPG_GS_KERNEL=2 selects the corner-per-wave colour kernel (k_gs0_colour_uc)
+// instead of the octet kernel (k_gs0_colour); both are parity-tested
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
-  const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
-  k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+  const char* e = std::getenv("SPG_GS_KERNEL");
+  if (e && std::atoi(e) == 2) {
+    k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+  } else {
+    const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
+    k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+  }
 }
 template <class T>
 void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
@@ -1476,11 +1605,12 @@ void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, co
 }
 template <class T>
 void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse) {
-  k_restrict<T><<<nblk((int64_t)C.nb * 64), TPB, 0, st>>>(C.nb, C.rsrc, C.flags, fine, coarse);
+  k_restrict_lds<T><<<(C.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(C.nb, C.rsrc, C.flags, fine, coarse);
 }
 template <class T>
 void launch_prolongate(hipStream_t st, const LevelDev& F, const T* coarse, T* fine) {
-  k_prolong<T><<<nblk((int64_t)F.nb * 64), TPB, 0, st>>>(F.nb, F.psrc, F.base, F.flags, coarse, fine);
+  k_prolong_lds<T><<<(F.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(F.nb, F.psrc, F.base, F.flags, coarse,
+                                                                              fine);
 }
 template <class T>
 void launch_zero(hipStream_t st, T* v, int64_t n) {

@@ -29,6 +29,8 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// relative residual reduction one FP32 MG-PCG pass is asked for at most
+static constexpr double kFp32PassReduction = 1e-6;
 // a bottom level above this many unknowns is refused (dense FP64 inverse)
 static constexpr int64_t kMaxBottomDofs = 60000;
 
@@ -456,8 +458,12 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
   if (std::isnan(r0)) rc = SPG_E_NAN;
   const int maxp = prm.max_passes < 1 ? 1 : prm.max_passes;
   for (int pass = 0; pass < maxp && !conv && rc == SPG_OK; ++pass) {
+    // per-pass target: an explicit inner reduction, or (inner_tolerance <= 0)
+    // the final target itself, but never asking an FP32 pass for more than
+    // kFp32PassReduction of its starting residual
     double tolp = target;
     if (prm.inner_tolerance > 0) tolp = std::max(prm.inner_tolerance * rn, 0.7 * target);
+    else if (prm.precision == 0) tolp = std::max(target, kFp32PassReduction * rn);
     int it = 0;
     double last = 0;
     int prc;
@@ -753,6 +759,77 @@ double Solver::bench_apply(int precision, int reps) {
   HIPCHK(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  return (double)ms * 1e-3 / reps;
+}
+
+double Solver::bench_kernel(int op, int level, int precision, int arg, int reps) {
+  if (level < 0 || level >= lay_.nlev) throw SpgError(SPG_E_INVALID, "level out of range");
+  const int B = lay_.nlev - 1;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float ms = 0.f;
+  auto body = [&](auto tag) {
+    using T = decltype(tag);
+    const LevelRT& R = lv_[level];
+    T* u = R.u<T>();
+    T* b = R.b<T>();
+    T* r = R.r<T>();
+    launch_zero<T>(st_, u, R.n3);
+    launch_zero<T>(st_, b, R.n3);
+    auto run = [&] {
+      switch (op) {
+        case SPG_OP_APPLY:
+        case SPG_OP_RESIDUAL:
+          if (level != 0) throw SpgError(SPG_E_INVALID, "apply is a level-0 operator");
+          launch_apply<T>(st_, R.d, E<T>(), dxT<T>(), u, b, r, op == SPG_OP_APPLY ? 2 : 1);
+          break;
+        case SPG_OP_GS_SWEEP:
+          if (level != 0 && !lay_.lv[level].is_explicit) throw SpgError(SPG_E_INVALID, "level has no smoother");
+          for (int c = 0; c < 8; ++c) {
+            if (level == 0) launch_gs0<T>(st_, R.d, E<T>(), dxT<T>(), u, b, sweep_colour(true, true, c));
+            else launch_st_gs<T>(st_, R.d, R.st<T>(), u, b, sweep_colour(false, true, c));
+          }
+          break;
+        case SPG_OP_GS_BOUNDARY:
+          if (level == 0) launch_gs0_boundary<T>(st_, R.d, E<T>(), dxT<T>(), u, b, arg, true);
+          else if (lay_.lv[level].is_explicit) launch_st_boundary<T>(st_, R.d, R.st<T>(), u, b, arg, true);
+          else throw SpgError(SPG_E_INVALID, "level has no smoother");
+          break;
+        case SPG_OP_RESTRICT:
+          if (level < 1) throw SpgError(SPG_E_INVALID, "restriction targets level >= 1");
+          launch_restrict<T>(st_, R.d, lv_[level - 1].r<T>(), b);
+          break;
+        case SPG_OP_PROLONGATE:
+          if (level >= B) throw SpgError(SPG_E_INVALID, "prolongation needs a coarser level");
+          launch_prolongate<T>(st_, R.d, lv_[level + 1].u<T>(), u);
+          break;
+        case SPG_OP_STENCIL_RES:
+          if (!lay_.lv[level].is_explicit) throw SpgError(SPG_E_INVALID, "level has no stencils");
+          launch_st_residual<T>(st_, R.d, R.st<T>(), u, b, r);
+          break;
+        case SPG_OP_VCYCLE:
+          vcycle<T>(lv_[0].b<T>(), lv_[0].r<T>());
+          break;
+        case SPG_OP_BOTTOM:
+          if (level != B) throw SpgError(SPG_E_INVALID, "bottom solve lives on the last level");
+          bottom<T>(b, u);
+          break;
+        default:
+          throw SpgError(SPG_E_INVALID, "unknown op");
+      }
+    };
+    run();  // warm
+    HIPCHK(hipEventRecord(e0, st_));
+    for (int i = 0; i < reps; ++i) run();
+    HIPCHK(hipEventRecord(e1, st_));
+  };
+  if (precision == 1) body(double());
+  else body(float());
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return (double)ms * 1e-3 / reps;
 }
 

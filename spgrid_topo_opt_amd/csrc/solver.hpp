@@ -77,6 +77,7 @@ class Solver {
   void run_kernel(int op, int level, int precision, int forward, const double* in, const double* b, double* out);
   void level_stencils(int level, double* out);
   double bench_apply(int precision, int reps);
+  double bench_kernel(int op, int level, int precision, int arg, int reps);
   const spg_config& config() const { return cfg_; }
   double setup_seconds() const { return setup_s_; }
   double device_bytes() const { return dev_bytes_; }

@@ -28,7 +28,7 @@ from . import _lib
 class FEM:
     """A GPU solver bound to one block topology, with solve settings."""
 
-    def __init__(self, prob, tolerance=1e-6, precision=0, inner_tolerance=1e-3, max_passes=30, max_iterations=500,
+    def __init__(self, prob, tolerance=1e-6, precision=0, inner_tolerance=0.0, max_passes=30, max_iterations=500,
                  device=0):
         self.prob = prob
         self.solver = _lib.Solver(prob, device=device)

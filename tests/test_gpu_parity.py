@@ -111,6 +111,23 @@ def test_gs_sweep_level0(blocky, prec, tol, forward, boundary):
     assert rel(got, want) < tol
 
 
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
+def test_gs_corner_colour_kernel(blocky, prec, tol, monkeypatch):
+    """the alternative level-0 colour kernel (a cell corner per wavefront, SPG_GS_KERNEL=2)"""
+    P = blocky
+    monkeypatch.setenv("SPG_GS_KERNEL", "2")
+    rng = np.random.default_rng(6)
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    u = rng.standard_normal((n, 3)) * 1e-6
+    u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    b = rng.standard_normal((n, 3))
+    for fwd in (True, False):
+        got = P.g.kernel(_lib.OP_GS_SWEEP, 0, u, b, precision=prec, forward=fwd)
+        want = P.from_o(0, P.o.gs_sweep(0, P.to_o(0, u), P.to_o(0, b), fwd, False))
+        assert rel(got, want) < tol
+
+
 @pytest.mark.parametrize("path", ["1", "2"])
 def test_boundary_fallback_paths(blocky, path, monkeypatch):
     """the global-memory boundary smoothers used when the boundary set does
