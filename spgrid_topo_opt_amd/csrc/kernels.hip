@@ -972,6 +972,176 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
     for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n + i];
   }
 }
+// ------------------------------------- level-0 matrix-free boundary phase
+// Level 0 needs no coefficient rows: boundary node i carries the moduli of
+// its 8 cells (bE[I*n + i], cell = node - I), the boundary indices of each
+// cell's 8 nodes (cnode, int16; -1 outside the set) and the inverse of its
+// 3x3 self block (bD, 6 entries + 2 pad).  Wave w takes cell corner I = w&7
+// of 64 nodes, so K0[I][J] is wave-uniform; the 8 corner rows meet in LDS
+// and the waves with I = 0 update the nodes.
+__global__ void k_bnd0_setup(int n, const int* __restrict__ list, const uint8_t* __restrict__ flags,
+                             const int* __restrict__ cslot, const double* __restrict__ E, double dx,
+                             double* __restrict__ bE, double* __restrict__ bD) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d[6] = {0, 0, 0, 0, 0, 0};
+  for (int I = 0; I < 8; ++I) {
+    const int sl = cslot[I * n + i];
+    const double e = sl >= 0 ? E[sl] : 0.0;
+    bE[I * n + i] = e;
+    const int kd = (I * 8 + I) * 9;
+    d[0] += e * cK0d[kd + 0];
+    d[1] += e * cK0d[kd + 4];
+    d[2] += e * cK0d[kd + 8];
+    d[3] += e * cK0d[kd + 1];
+    d[4] += e * cK0d[kd + 2];
+    d[5] += e * cK0d[kd + 5];
+  }
+  for (int q = 0; q < 6; ++q) d[q] *= dx;
+  dirichlet_inverse(d, flags[list[i]]);
+  for (int q = 0; q < 6; ++q) bD[i * 8 + q] = d[q];
+  bD[i * 8 + 6] = 0.0;
+  bD[i * 8 + 7] = 0.0;
+}
+
+template <class T>
+struct Bnd0Fetch {
+  T e;
+  int c[4];  // 8 int16 node indices, two per int
+  T r[3];
+  T di[6];
+  int i;
+  bool live;
+};
+
+static constexpr int B0_THREADS = 1024;
+static constexpr int B0_NODES = 128;  // nodes per pass (2 groups of 64 x 8 corners)
+template <class T>
+__global__ void __launch_bounds__(B0_THREADS) k_bnd0_sweeps(int n, BndOff L, ColourOrder ord, int sweeps,
+                                                            const int* __restrict__ list,
+                                                            const uint8_t* __restrict__ flags,
+                                                            const T* __restrict__ bE,
+                                                            const int16_t* __restrict__ cnode,
+                                                            const T* __restrict__ bD, const T* __restrict__ bt,
+                                                            T dx, T* __restrict__ u) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n1 = n + 1;                                 // entry n of each component is a zero sentinel
+  T* ub = reinterpret_cast<T*>(smem_raw);               // [3][n+1]
+  T* pf = ub + 3 * n1;                                  // [2][8][3][128]
+  uint8_t* fb = reinterpret_cast<uint8_t*>(pf + 2 * 8 * 3 * B0_NODES);  // [n]
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int I = __builtin_amdgcn_readfirstlane(w & 7);
+  const int j = (w >> 3) * 64 + lane;  // node within the pass
+  for (int i = tid; i < n; i += B0_THREADS) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ub[c * n1 + i] = u[vidx(blk, c, l)];
+    fb[i] = flags[node];
+  }
+  if (tid < 3) ub[tid * n1 + n] = T(0);
+  auto fetch = [&](Bnd0Fetch<T>& F, const BndCursor& k) {
+    const int i = L.off[k.c] + k.p * B0_NODES + j;
+    F.i = i;
+    F.live = i < L.off[k.c + 1];
+    if (F.live) {
+      F.e = bE[I * n + i];
+      const int4 a = *reinterpret_cast<const int4*>(cnode + ((size_t)I * n + i) * 8);
+      F.c[0] = a.x;
+      F.c[1] = a.y;
+      F.c[2] = a.z;
+      F.c[3] = a.w;
+    } else {
+      F.e = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) F.c[q] = -1;
+    }
+    if (F.live && I == 0) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) F.r[q] = bt[i * 4 + q];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) F.di[q] = bD[i * 8 + q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) F.r[q] = T(0);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) F.di[q] = T(0);
+    }
+  };
+  auto pass = [&](const Bnd0Fetch<T>& F, int buf, bool end) {
+    // row of corner I over the node's boundary neighbours (outside ones read the sentinel)
+    T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+    for (int J = 0; J < 8; ++J) {
+      const int wv = F.c[J >> 1];
+      int q = (J & 1) ? (wv >> 16) : (int)(short)(wv & 0xffff);
+      q = q < 0 ? n : q;
+      const T v0 = ub[q], v1 = ub[n1 + q], v2 = ub[2 * n1 + q];
+      const int kb = (I * 8 + J) * 9;
+      a0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
+      a1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
+      a2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
+    }
+    T* P = pf + buf * (8 * 3 * B0_NODES);
+    P[(I * 3 + 0) * B0_NODES + j] = F.e * a0;
+    P[(I * 3 + 1) * B0_NODES + j] = F.e * a1;
+    P[(I * 3 + 2) * B0_NODES + j] = F.e * a2;
+    __syncthreads();
+    if (I == 0 && F.live) {
+      T f[3];
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        T acc = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc += P[(c * 3 + v) * B0_NODES + j];
+        f[v] = dx * acc;
+      }
+      const int i = F.i;
+      const uint8_t fl = fb[i];
+      if (fl & 1) {
+        const T r0 = F.r[0] - f[0], r1 = F.r[1] - f[1], r2 = F.r[2] - f[2];
+        const T u0 = ub[i] + F.di[0] * r0 + F.di[3] * r1 + F.di[4] * r2;
+        const T u1 = ub[n1 + i] + F.di[3] * r0 + F.di[1] * r1 + F.di[5] * r2;
+        const T u2 = ub[2 * n1 + i] + F.di[4] * r0 + F.di[5] * r1 + F.di[2] * r2;
+        ub[i] = (fl & 2) ? T(0) : u0;
+        ub[n1 + i] = (fl & 4) ? T(0) : u1;
+        ub[2 * n1 + i] = (fl & 8) ? T(0) : u2;
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+          if (fl & (2 << cc)) ub[cc * n1 + i] = T(0);
+      }
+    }
+    if (end) __syncthreads();
+  };
+  BndCursor cur{0, 0, 0, ord.at(0)};
+  Bnd0Fetch<T> X, Y;
+  fetch(X, cur);
+  __syncthreads();
+  int buf = 0;
+  while (true) {
+    bool end;
+    BndCursor nx = bnd_advance<B0_NODES>(cur, L, ord, end);
+    bool more = nx.sw < sweeps;
+    fetch(Y, more ? nx : cur);
+    pass(X, buf, end);
+    buf ^= 1;
+    if (!more) break;
+    cur = nx;
+    nx = bnd_advance<B0_NODES>(cur, L, ord, end);
+    more = nx.sw < sweeps;
+    fetch(X, more ? nx : cur);
+    pass(Y, buf, end);
+    buf ^= 1;
+    if (!more) break;
+    cur = nx;
+  }
+  for (int i = tid; i < n; i += B0_THREADS) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n1 + i];
+  }
+}
+
 // workgroup size of the boundary phase: two register sets of BndFetch per lane
 template <class T>
 struct BndThreads {
@@ -1497,8 +1667,9 @@ static BndLists bnd_lists(const LevelDev& L) {
 static constexpr int kFusedBoundaryMax = 4096;
 // SPG_BOUNDARY_PATH selects the boundary smoother for tests: 0 (default)
 // automatic, 1 the single-workgroup global-memory kernel, 2 one launch per
-// colour step.  Paths 1 and 2 are the fallbacks for boundary sets too large
-// for the LDS-resident system.
+// colour step, 3 the coefficient-row LDS system also on level 0.  Paths 1
+// and 2 are the fallbacks for boundary sets too large for the LDS-resident
+// systems.
 static int boundary_path() {
   const char* e = std::getenv("SPG_BOUNDARY_PATH");
   return e ? std::atoi(e) : 0;
@@ -1539,6 +1710,46 @@ static bool launch_bnd_phase(hipStream_t st, const LevelDev& L, T* u, const T* b
                                           u);
   return true;
 }
+template <class T>
+struct Bnd0Ptr;
+template <>
+struct Bnd0Ptr<float> {
+  static const float* E(const LevelDev& L) { return L.b0E32; }
+  static const float* D(const LevelDev& L) { return L.b0D32; }
+};
+template <>
+struct Bnd0Ptr<double> {
+  static const double* E(const LevelDev& L) { return L.b0E64; }
+  static const double* D(const LevelDev& L) { return L.b0D64; }
+};
+// level-0 boundary phase on the matrix-free LDS-resident system; false if
+// the tables are absent or the LDS does not hold it
+template <class T>
+static bool launch_bnd0_phase(hipStream_t st, const LevelDev& L, T dx, T* u, const T* b, int sweeps, bool forward) {
+  const int n = L.bnd_off[8];
+  const size_t lds = (size_t)(3 * (n + 1) + 2 * 8 * 3 * B0_NODES) * sizeof(T) + (size_t)n;
+  const T* A = BndPtr<T>::A(L);
+  T* bt = BndPtr<T>::bt(L);
+  const T* bE = Bnd0Ptr<T>::E(L);
+  const T* bD = Bnd0Ptr<T>::D(L);
+  if (!A || !bt || !bE || !bD || !L.b0node || !L.bslot || lds > kMaxLds) return false;
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute((const void*)k_bnd0_sweeps<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+    return true;
+  }();
+  (void)attr;
+  k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt);
+  BndOff o;
+  for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
+  k_bnd0_sweeps<T><<<1, B0_THREADS, lds, st>>>(n, o, colour_order(true, forward), sweeps, L.bnd, L.flags, bE,
+                                               L.b0node, bD, bt, dx, u);
+  return true;
+}
+void launch_bnd0_setup(hipStream_t st, const LevelDev& L, const int* cslot, const double* E, double dx, double* bE,
+                       double* bD) {
+  const int n = L.bnd_off[8];
+  if (n > 0) k_bnd0_setup<<<nblk(n), TPB, 0, st>>>(n, L.bnd, L.flags, cslot, E, dx, bE, bD);
+}
 void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double dx, double* A) {
   const int n = L.bnd_off[8];
   if (n > 0) k_bnd_coef0<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, L.nbr, E, dx, A);
@@ -1565,7 +1776,8 @@ void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T*
                          bool forward) {
   if (sweeps <= 0 || L.bnd_off[8] == 0) return;
   const int path = boundary_path();
-  if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
+  if (path == 0 && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
+  if ((path == 0 || path == 3) && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
   const ColourOrder ord = colour_order(true, forward);
   if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {
     k_gs0_boundary<T><<<1, BND_THREADS, 0, st>>>(bnd_lists(L), ord, sweeps, L.nbr, L.flags, E, dx, u, b);

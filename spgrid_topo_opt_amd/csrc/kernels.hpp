@@ -30,6 +30,13 @@ struct LevelDev {
   const double* bA64 = nullptr;
   float* bt32 = nullptr;
   double* bt64 = nullptr;
+  // level-0 matrix-free boundary system: cell-node indices [8*n*8] (int16),
+  // cell moduli [8*n] and inverse self blocks [n*8] per precision
+  const int16_t* b0node = nullptr;
+  const float* b0E32 = nullptr;
+  const double* b0E64 = nullptr;
+  const float* b0D32 = nullptr;
+  const double* b0D64 = nullptr;
 };
 
 // Colour of the i-th step of a Gauss-Seidel sweep: level 0 runs the colours
@@ -118,6 +125,9 @@ void launch_stencil_to_float(hipStream_t st, const double* a, float* b, int64_t 
 // coefficient rows of the boundary nodes (FP64): level 0 from E and K0 (times
 // dx), explicit levels from the stencils
 void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double dx, double* A);
+// level-0 matrix-free boundary system (FP64): cell moduli and inverse self blocks
+void launch_bnd0_setup(hipStream_t st, const LevelDev& L, const int* cslot, const double* E, double dx, double* bE,
+                       double* bD);
 void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A);
 // bottom
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,

@@ -371,6 +371,30 @@ Layout build_layout(const ProblemDesc& p) {
         else D.bnd_slot[e] = slot;
       }
     }
+    if (l == 0 && !all.empty() && all.size() < 32768) {
+      const size_t n = all.size();
+      D.bnd_cslot.assign(8 * n, -1);
+      D.bnd_cnode.assign(64 * n, (int16_t)-1);
+      auto slot_of = [&](int32_t g, int a, int b, int c) -> int32_t {
+        const int rs = (a < 0 ? 0 : (a > 3 ? 2 : 1)) * 9 + (b < 0 ? 0 : (b > 3 ? 2 : 1)) * 3 + (c < 0 ? 0 : (c > 3 ? 2 : 1));
+        const int32_t nb = D.nbr[(size_t)g * 27 + rs];
+        return nb < 0 ? -1 : nb * 64 + (((a & 3) << 4) | ((b & 3) << 2) | (c & 3));
+      };
+      for (size_t i = 0; i < n; ++i) {
+        const int32_t g = all[i] >> 6, q = all[i] & 63;
+        const int li = q >> 4, lj = (q >> 2) & 3, lk = q & 3;
+        for (int I = 0; I < 8; ++I) {
+          const int cx = li - ((I >> 2) & 1), cy = lj - ((I >> 1) & 1), cz = lk - (I & 1);
+          D.bnd_cslot[(size_t)I * n + i] = slot_of(g, cx, cy, cz);
+          for (int J = 0; J < 8; ++J) {
+            const int32_t sl = slot_of(g, cx + ((J >> 2) & 1), cy + ((J >> 1) & 1), cz + (J & 1));
+            if (sl < 0) continue;
+            auto it = pos.find(sl);
+            if (it != pos.end()) D.bnd_cnode[((size_t)I * n + i) * 8 + J] = (int16_t)it->second;
+          }
+        }
+      }
+    }
   }
   // transfers
   for (int l = 0; l < out.nlev; ++l) {

@@ -101,6 +101,12 @@ struct LevelDesc {
   // the boundary list (bnd_ci) or, for a neighbour outside it, its slot
   // blk*64+l (bnd_slot); -1 where absent or not applicable.
   std::vector<int32_t> bnd_ci, bnd_slot;
+  // level-0 matrix-free boundary system: for corner I and boundary node i,
+  // bnd_cslot[I*n + i] is the slot of cell (node - I) (-1 if absent) and
+  // bnd_cnode[(I*n + i)*8 + J] the boundary index of that cell's node J
+  // (-1 outside the boundary set)
+  std::vector<int32_t> bnd_cslot;
+  std::vector<int16_t> bnd_cnode;
   int64_t n_active = 0;
   bool is_explicit = false;
   // explicit levels: Galerkin data

@@ -128,6 +128,19 @@ void Solver::upload_layout() {
       R.bt64.alloc(bnd.size() * 4 * 8);
       R.bt32.alloc(bnd.size() * 4 * 4);
     }
+    if (!D.bnd_cslot.empty()) {
+      upload(R.b0cslot, D.bnd_cslot);
+      upload(R.b0node, D.bnd_cnode);
+      R.b0E64.alloc(D.bnd_cslot.size() * 8);
+      R.b0E32.alloc(D.bnd_cslot.size() * 4);
+      R.b0D64.alloc(bnd.size() * 8 * 8);
+      R.b0D32.alloc(bnd.size() * 8 * 4);
+      R.d.b0node = R.b0node.as<int16_t>();
+      R.d.b0E64 = R.b0E64.as<double>();
+      R.d.b0E32 = R.b0E32.as<float>();
+      R.d.b0D64 = R.b0D64.as<double>();
+      R.d.b0D32 = R.b0D32.as<float>();
+    }
     R.d.bci = R.bci.as<int>();
     R.d.bslot = R.bslot.as<int>();
     R.d.bA64 = R.bA64.as<double>();
@@ -252,6 +265,12 @@ void Solver::build_boundary_rows() {
     LevelRT& R = lv_[l];
     const int n = R.d.bnd_off[8];
     if (n == 0 || !R.bA64.p) continue;
+    if (l == 0 && R.b0cslot.p) {
+      launch_bnd0_setup(st_, R.d, R.b0cslot.as<int>(), E64_.as<double>(), cfg_.dx, R.b0E64.as<double>(),
+                        R.b0D64.as<double>());
+      launch_cast_d2f(st_, R.b0E64.as<double>(), R.b0E32.as<float>(), (int64_t)n * 8);
+      launch_cast_d2f(st_, R.b0D64.as<double>(), R.b0D32.as<float>(), (int64_t)n * 8);
+    }
     if (l == 0) launch_bnd_coef0(st_, R.d, E64_.as<double>(), cfg_.dx, R.bA64.as<double>());
     else if (lay_.lv[l].is_explicit) launch_bnd_coef_st(st_, R.d, R.st64.as<double>(), R.bA64.as<double>());
     else continue;

@@ -128,10 +128,11 @@ def test_gs_corner_colour_kernel(blocky, prec, tol, monkeypatch):
         assert rel(got, want) < tol
 
 
-@pytest.mark.parametrize("path", ["1", "2"])
+@pytest.mark.parametrize("path", ["1", "2", "3"])
 def test_boundary_fallback_paths(blocky, path, monkeypatch):
-    """the global-memory boundary smoothers used when the boundary set does
-    not fit in LDS: same result as the oracle on every smoothed level"""
+    """the other boundary smoothers (1, 2: global memory, used when the
+    boundary set does not fit in LDS; 3: coefficient rows on level 0 too):
+    same result as the oracle on every smoothed level"""
     P = blocky
     monkeypatch.setenv("SPG_BOUNDARY_PATH", path)
     rng = np.random.default_rng(11)
