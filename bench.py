@@ -142,7 +142,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("k_apply_lds<float, 2>", nn),
         },
         "cpu_baseline": cpu,
     }
@@ -150,6 +150,23 @@ def main():
 
 
 COPY_BYTES = 512 << 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_shell256.json")
+
+
+def measured_traffic(kernel, active_nodes):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    same counter reads for a copy of known size); None when no summary for
+    this workload is present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if k is None or t.get("active_nodes") != active_nodes:
+        return None
+    return k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"]
 
 
 def traffic_probe(s, prob):

@@ -37,6 +37,12 @@ void upload_k0(const double* K0, hipStream_t st) {
   hipMemcpyToSymbol(HIP_SYMBOL(cK0f), f, sizeof(float) * 576, 0, hipMemcpyHostToDevice);
 }
 
+// fused multiply-add in the working precision; row sums are written as FMA
+// chains onto their accumulator (a*b + c*d + e*f + acc would cost a multiply
+// and an add per term triple more)
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 static constexpr int TPB = 256;
 static inline unsigned nblk(int64_t n, int t = TPB) { return (unsigned)((n + t - 1) / t); }
 
@@ -105,13 +111,19 @@ __device__ inline void row0(const int* __restrict__ nbr, const T* __restrict__ E
       const int jx = (J >> 2) & 1, jy = (J >> 1) & 1, jz = J & 1;
       const int s = (jx - ix + 1) * 9 + (jy - iy + 1) * 3 + (jz - iz + 1);
       const int kb = (I * 8 + J) * 9;
-      t0 += KT<T>::k(kb + 0) * uu[s][0] + KT<T>::k(kb + 1) * uu[s][1] + KT<T>::k(kb + 2) * uu[s][2];
-      t1 += KT<T>::k(kb + 3) * uu[s][0] + KT<T>::k(kb + 4) * uu[s][1] + KT<T>::k(kb + 5) * uu[s][2];
-      t2 += KT<T>::k(kb + 6) * uu[s][0] + KT<T>::k(kb + 7) * uu[s][1] + KT<T>::k(kb + 8) * uu[s][2];
+      t0 = fmaT(KT<T>::k(kb + 0), uu[s][0], t0);
+      t0 = fmaT(KT<T>::k(kb + 1), uu[s][1], t0);
+      t0 = fmaT(KT<T>::k(kb + 2), uu[s][2], t0);
+      t1 = fmaT(KT<T>::k(kb + 3), uu[s][0], t1);
+      t1 = fmaT(KT<T>::k(kb + 4), uu[s][1], t1);
+      t1 = fmaT(KT<T>::k(kb + 5), uu[s][2], t1);
+      t2 = fmaT(KT<T>::k(kb + 6), uu[s][0], t2);
+      t2 = fmaT(KT<T>::k(kb + 7), uu[s][1], t2);
+      t2 = fmaT(KT<T>::k(kb + 8), uu[s][2], t2);
     }
-    f0 += e * t0;
-    f1 += e * t1;
-    f2 += e * t2;
+    f0 = fmaT(e, t0, f0);
+    f1 = fmaT(e, t1, f1);
+    f2 = fmaT(e, t2, f2);
     if (DIAG) {
       const int kd = (I * 8 + I) * 9;
       d0 += e * KT<T>::k(kd + 0);
@@ -206,13 +218,19 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
       const int jx = (J >> 2) & 1, jy = (J >> 1) & 1, jz = J & 1;
       const int sp = (jx - ix + 1) * 9 + (jy - iy + 1) * 3 + (jz - iz + 1);
       const int kb = (I * 8 + J) * 9;
-      t0 += KT<T>::k(kb + 0) * uu[sp][0] + KT<T>::k(kb + 1) * uu[sp][1] + KT<T>::k(kb + 2) * uu[sp][2];
-      t1 += KT<T>::k(kb + 3) * uu[sp][0] + KT<T>::k(kb + 4) * uu[sp][1] + KT<T>::k(kb + 5) * uu[sp][2];
-      t2 += KT<T>::k(kb + 6) * uu[sp][0] + KT<T>::k(kb + 7) * uu[sp][1] + KT<T>::k(kb + 8) * uu[sp][2];
+      t0 = fmaT(KT<T>::k(kb + 0), uu[sp][0], t0);
+      t0 = fmaT(KT<T>::k(kb + 1), uu[sp][1], t0);
+      t0 = fmaT(KT<T>::k(kb + 2), uu[sp][2], t0);
+      t1 = fmaT(KT<T>::k(kb + 3), uu[sp][0], t1);
+      t1 = fmaT(KT<T>::k(kb + 4), uu[sp][1], t1);
+      t1 = fmaT(KT<T>::k(kb + 5), uu[sp][2], t1);
+      t2 = fmaT(KT<T>::k(kb + 6), uu[sp][0], t2);
+      t2 = fmaT(KT<T>::k(kb + 7), uu[sp][1], t2);
+      t2 = fmaT(KT<T>::k(kb + 8), uu[sp][2], t2);
     }
-    f0 += e * t0;
-    f1 += e * t1;
-    f2 += e * t2;
+    f0 = fmaT(e, t0, f0);
+    f1 = fmaT(e, t1, f1);
+    f2 = fmaT(e, t2, f2);
   }
   if (!live) return;
   const T f[3] = {dx * f0, dx * f1, dx * f2};
@@ -319,9 +337,15 @@ __device__ __forceinline__ void corner_row(int I, T e, const LoadU& U, T& f0, T&
     v1 *= sy;
     v2 *= sz;
     const int kb = Jp * 9;
-    a0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
-    a1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
-    a2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
+    a0 = fmaT(KT<T>::k(kb + 0), v0, a0);
+    a0 = fmaT(KT<T>::k(kb + 1), v1, a0);
+    a0 = fmaT(KT<T>::k(kb + 2), v2, a0);
+    a1 = fmaT(KT<T>::k(kb + 3), v0, a1);
+    a1 = fmaT(KT<T>::k(kb + 4), v1, a1);
+    a1 = fmaT(KT<T>::k(kb + 5), v2, a1);
+    a2 = fmaT(KT<T>::k(kb + 6), v0, a2);
+    a2 = fmaT(KT<T>::k(kb + 7), v1, a2);
+    a2 = fmaT(KT<T>::k(kb + 8), v2, a2);
   }
   f0 = sx * e * a0;
   f1 = sy * e * a1;
@@ -452,9 +476,15 @@ __global__ void __launch_bounds__(512) k_gs0_colour_uc(int nb, int colour, const
       const int h = bl * 125 + ((cx + ((J >> 2) & 1)) * 5 + (cy + ((J >> 1) & 1))) * 5 + (cz + (J & 1));
       const T v0 = hu[0][h], v1 = hu[1][h], v2 = hu[2][h];
       const int kb = (I * 8 + J) * 9;
-      t0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
-      t1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
-      t2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
+      t0 = fmaT(KT<T>::k(kb + 0), v0, t0);
+      t0 = fmaT(KT<T>::k(kb + 1), v1, t0);
+      t0 = fmaT(KT<T>::k(kb + 2), v2, t0);
+      t1 = fmaT(KT<T>::k(kb + 3), v0, t1);
+      t1 = fmaT(KT<T>::k(kb + 4), v1, t1);
+      t1 = fmaT(KT<T>::k(kb + 5), v2, t1);
+      t2 = fmaT(KT<T>::k(kb + 6), v0, t2);
+      t2 = fmaT(KT<T>::k(kb + 7), v1, t2);
+      t2 = fmaT(KT<T>::k(kb + 8), v2, t2);
     }
     pf[I][0][lane] = e * t0;
     pf[I][1][lane] = e * t1;
@@ -483,6 +513,84 @@ __global__ void __launch_bounds__(512) k_gs0_colour_uc(int nb, int colour, const
   const int hn = bl * 125 + (nx * 5 + ny) * 5 + nz;
   T d[6] = {dx * d0, dx * d1, dx * d2, dx * d3, dx * d4, dx * d5};
   gs_node_update<T>(flags[blk * 64 + l], d, dx * f0, dx * f1, dx * f2, blk, l, hu[0][hn], hu[1][hn], hu[2][hn], b, u);
+}
+
+// sum over the 4 lanes of an aligned quad (lane bits 0..1)
+template <class T>
+__device__ __forceinline__ T quad_sum(T x) {
+  x += dpp_t<0xB1>(x);  // partner lane ^ 1
+  x += dpp_t<0x4E>(x);  // partner lane ^ 2
+  return x;
+}
+
+// One colour of the level-0 sweep with 4 lanes per node: a wavefront owns
+// the colour-c nodes of 2 blocks; lane (k, q) adds the rows of colour node k
+// in its cells with corners I = q and q|4 (the reflection form of
+// corner_row), the quad sums, and lane q = 0 updates the node.  A 256-thread
+// workgroup stages the 5^3 halos of its 8 blocks.  Same race argument as
+// k_gs0_colour.
+static constexpr int Q_BLOCKS = 8;
+template <class T>
+__global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const int* __restrict__ nbr,
+                                                     const uint8_t* __restrict__ flags, const T* __restrict__ E,
+                                                     T dx, T* __restrict__ u, const T* __restrict__ b) {
+  __shared__ T hu[Q_BLOCKS][3][128];
+  __shared__ int ht[Q_BLOCKS][27];
+  const int tid = threadIdx.x;
+  const int blk0 = xcd_chunk(blockIdx.x, gridDim.x) * Q_BLOCKS;
+  const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
+  if (tid < Q_BLOCKS * 27) {
+    const int bl = tid / 27, sp = tid - bl * 27, blk = blk0 + bl;
+    ht[bl][sp] = blk < nb ? (sp == 13 ? blk : nbr[blk * 27 + sp]) : -1;
+  }
+  __syncthreads();
+  for (int h = tid; h < Q_BLOCKS * 125; h += 256) {
+    const int bl = h / 125, hh = h - bl * 125;
+    const int x = px - 1 + hh / 25, y = py - 1 + (hh / 5) % 5, z = pz - 1 + hh % 5;
+    const int bb = ht[bl][ring_spoke(x, y, z)];
+    const int q = loc3(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hu[bl][c][hh] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+  }
+  __syncthreads();
+  const int bl = tid >> 5, k = (tid >> 2) & 7, q = tid & 3;
+  const int blk = blk0 + bl;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;  // halo coords
+  const T* H = &hu[bl][0][0];
+  T f0 = 0, f1 = 0, f2 = 0, ee[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int I = q | (r << 2);
+    const int cx = nx - ((I >> 2) & 1), cy = ny - ((I >> 1) & 1), cz = nz - (I & 1);  // cell corner (halo)
+    const int ex = px - 1 + cx, ey = py - 1 + cy, ez = pz - 1 + cz;                   // block coords
+    const int eb = ht[bl][ring_spoke(ex, ey, ez)];
+    const T e = eb >= 0 ? E[eb * 64 + loc3(ex, ey, ez)] : T(0);
+    ee[r] = e;
+    auto U = [&](int J, T& v0, T& v1, T& v2) {
+      const int h = ((cx + ((J >> 2) & 1)) * 5 + (cy + ((J >> 1) & 1))) * 5 + (cz + (J & 1));
+      v0 = H[h];
+      v1 = H[128 + h];
+      v2 = H[256 + h];
+    };
+    T g0, g1, g2;
+    corner_row<T>(I, e, U, g0, g1, g2);
+    f0 += g0;
+    f1 += g1;
+    f2 += g2;
+  }
+  // self block: corners q and q|4 differ in the x mirror only
+  const T sy = (q & 2) ? T(-1) : T(1), sz = (q & 1) ? T(-1) : T(1);
+  const T es = ee[0] + ee[1], ed = ee[0] - ee[1];
+  f0 = dx * quad_sum(f0);
+  f1 = dx * quad_sum(f1);
+  f2 = dx * quad_sum(f2);
+  const T S = dx * quad_sum(es), Sxy = dx * quad_sum(sy * ed), Sxz = dx * quad_sum(sz * ed),
+          Syz = dx * quad_sum(sy * sz * es);
+  if (q != 0 || blk >= nb) return;
+  T d[6] = {KT<T>::k(0) * S, KT<T>::k(4) * S, KT<T>::k(8) * S, KT<T>::k(1) * Sxy, KT<T>::k(2) * Sxz, KT<T>::k(5) * Syz};
+  const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+  const int hn = (nx * 5 + ny) * 5 + nz;
+  gs_node_update<T>(flags[blk * 64 + l], d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], b, u);
 }
 
 // Level-0 update of list node `s` (blk*64+l) by the 8 lanes of an octet,
@@ -581,9 +689,15 @@ __device__ __forceinline__ void st_octet_row(int k, int g, const T* __restrict__
     T m[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) m[q] = t < 3 ? S[(t * 9 + q) * 64 + k * 8 + g] : S[1728 + q * 24 + k * 3 + g];
-    a0 += m[0] * x + m[1] * y + m[2] * z;
-    a1 += m[3] * x + m[4] * y + m[5] * z;
-    a2 += m[6] * x + m[7] * y + m[8] * z;
+    a0 = fmaT(m[0], x, a0);
+    a0 = fmaT(m[1], y, a0);
+    a0 = fmaT(m[2], z, a0);
+    a1 = fmaT(m[3], x, a1);
+    a1 = fmaT(m[4], y, a1);
+    a1 = fmaT(m[5], z, a1);
+    a2 = fmaT(m[6], x, a2);
+    a2 = fmaT(m[7], y, a2);
+    a2 = fmaT(m[8], z, a2);
     if (t == 1) {
 #pragma unroll
       for (int q = 0; q < 9; ++q) c9[q] = m[q];
@@ -765,9 +879,15 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
     if (sl >= 0) {
       const T x = u[vidx(sl >> 6, 0, sl & 63)], y = u[vidx(sl >> 6, 1, sl & 63)], z = u[vidx(sl >> 6, 2, sl & 63)];
       const T* M = A + (size_t)i * 288 + g * 36 + t * 9;
-      a0 += M[0] * x + M[1] * y + M[2] * z;
-      a1 += M[3] * x + M[4] * y + M[5] * z;
-      a2 += M[6] * x + M[7] * y + M[8] * z;
+      a0 = fmaT(M[0], x, a0);
+      a0 = fmaT(M[1], y, a0);
+      a0 = fmaT(M[2], z, a0);
+      a1 = fmaT(M[3], x, a1);
+      a1 = fmaT(M[4], y, a1);
+      a1 = fmaT(M[5], z, a1);
+      a2 = fmaT(M[6], x, a2);
+      a2 = fmaT(M[7], y, a2);
+      a2 = fmaT(M[8], z, a2);
     }
   }
   a0 = octet_sum(a0);
@@ -876,9 +996,15 @@ __device__ __forceinline__ void bnd_pass(const BndFetch<T>& F, int g, int n, T* 
     const int j = F.j[t];
     if (j >= 0) {
       const T x = ub[j], y = ub[n + j], z = ub[2 * n + j];
-      a0 += F.m[t * 9 + 0] * x + F.m[t * 9 + 1] * y + F.m[t * 9 + 2] * z;
-      a1 += F.m[t * 9 + 3] * x + F.m[t * 9 + 4] * y + F.m[t * 9 + 5] * z;
-      a2 += F.m[t * 9 + 6] * x + F.m[t * 9 + 7] * y + F.m[t * 9 + 8] * z;
+      a0 = fmaT(F.m[t * 9 + 0], x, a0);
+      a0 = fmaT(F.m[t * 9 + 1], y, a0);
+      a0 = fmaT(F.m[t * 9 + 2], z, a0);
+      a1 = fmaT(F.m[t * 9 + 3], x, a1);
+      a1 = fmaT(F.m[t * 9 + 4], y, a1);
+      a1 = fmaT(F.m[t * 9 + 5], z, a1);
+      a2 = fmaT(F.m[t * 9 + 6], x, a2);
+      a2 = fmaT(F.m[t * 9 + 7], y, a2);
+      a2 = fmaT(F.m[t * 9 + 8], z, a2);
     }
   }
   a0 = octet_sum(a0);
@@ -1077,9 +1203,15 @@ __global__ void __launch_bounds__(B0_THREADS) k_bnd0_sweeps(int n, BndOff L, Col
       q = q < 0 ? n : q;
       const T v0 = ub[q], v1 = ub[n1 + q], v2 = ub[2 * n1 + q];
       const int kb = (I * 8 + J) * 9;
-      a0 += KT<T>::k(kb + 0) * v0 + KT<T>::k(kb + 1) * v1 + KT<T>::k(kb + 2) * v2;
-      a1 += KT<T>::k(kb + 3) * v0 + KT<T>::k(kb + 4) * v1 + KT<T>::k(kb + 5) * v2;
-      a2 += KT<T>::k(kb + 6) * v0 + KT<T>::k(kb + 7) * v1 + KT<T>::k(kb + 8) * v2;
+      a0 = fmaT(KT<T>::k(kb + 0), v0, a0);
+      a0 = fmaT(KT<T>::k(kb + 1), v1, a0);
+      a0 = fmaT(KT<T>::k(kb + 2), v2, a0);
+      a1 = fmaT(KT<T>::k(kb + 3), v0, a1);
+      a1 = fmaT(KT<T>::k(kb + 4), v1, a1);
+      a1 = fmaT(KT<T>::k(kb + 5), v2, a1);
+      a2 = fmaT(KT<T>::k(kb + 6), v0, a2);
+      a2 = fmaT(KT<T>::k(kb + 7), v1, a2);
+      a2 = fmaT(KT<T>::k(kb + 8), v2, a2);
     }
     T* P = pf + buf * (8 * 3 * B0_NODES);
     P[(I * 3 + 0) * B0_NODES + j] = F.e * a0;
@@ -1759,13 +1891,17 @@ void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, dou
   if (n > 0) k_bnd_coef_st<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, ST, A);
 }
 
-// SPG_GS_KERNEL=2 selects the corner-per-wave colour kernel (k_gs0_colour_uc)
-// instead of the octet kernel (k_gs0_colour); both are parity-tested
+// SPG_GS_KERNEL selects the level-0 colour kernel: 0 (default) the octet
+// kernel k_gs0_colour, 2 the corner-per-wave k_gs0_colour_uc, 3 the
+// quad-lane k_gs0_colour4; all are parity-tested
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
   const char* e = std::getenv("SPG_GS_KERNEL");
-  if (e && std::atoi(e) == 2) {
+  const int kind = e ? std::atoi(e) : 0;
+  if (kind == 2) {
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+  } else if (kind == 3) {
+    k_gs0_colour4<T><<<(L.nb + Q_BLOCKS - 1) / Q_BLOCKS, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   } else {
     const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
