@@ -170,19 +170,26 @@ def measured_traffic(kernel, active_nodes):
 
 
 def traffic_probe(s, prob):
-    """Five launches of the level-0 FP32 apply and five of a plain device copy
-    of COPY_BYTES (read COPY_BYTES + write COPY_BYTES).  Under
-    `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) the copy's per-dispatch
-    counter gives the KiB->bytes scale applied to the apply's counter."""
+    """A few launches of the hot FP32 kernels (level-0 apply, one level-0
+    Gauss-Seidel sweep, the level-0 and level-2 boundary phases) and three of
+    a plain device copy of COPY_BYTES (read COPY_BYTES + write COPY_BYTES).
+    Under `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) the copy's
+    per-dispatch counter gives the KiB->bytes scale applied to the others."""
     import torch
 
+    from spgrid_topo_opt_amd import _lib
+
     s.bench_apply(0, 5)
+    s.bench_kernel(_lib.OP_GS_SWEEP, 0, 0, 0, 1)
+    s.bench_kernel(_lib.OP_GS_BOUNDARY, 0, 0, prob.boundary_iters, 1)
+    if s.levels > 3:
+        s.bench_kernel(_lib.OP_GS_BOUNDARY, 2, 0, prob.boundary_iters, 1)
     x = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
     y = torch.empty_like(x)
-    for _ in range(5):
+    for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    log(f"[bench] traffic probe done: FP32 apply x5, copy of {COPY_BYTES} B x5")
+    log(f"[bench] traffic probe done: FP32 apply x5+1, GS sweep, boundary phases, copy of {COPY_BYTES} B x3")
 
 
 def cpu_baseline(prob, gpu_iterations, budget_s):

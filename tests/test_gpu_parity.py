@@ -111,11 +111,11 @@ def test_gs_sweep_level0(blocky, prec, tol, forward, boundary):
     assert rel(got, want) < tol
 
 
-@pytest.mark.parametrize("kind", ["2", "3"])
+@pytest.mark.parametrize("kind", ["1", "2", "3"])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
-def test_gs_alternative_colour_kernels(blocky, prec, tol, kind, monkeypatch):
-    """the alternative level-0 colour kernels (SPG_GS_KERNEL=2: a cell corner
-    per wavefront; 3: four lanes per node)"""
+def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
+    """every level-0 colour kernel (SPG_GS_KERNEL=1: eight lanes per node;
+    2: a cell corner per wavefront; 3: four lanes per node)"""
     P = blocky
     monkeypatch.setenv("SPG_GS_KERNEL", kind)
     rng = np.random.default_rng(6)
