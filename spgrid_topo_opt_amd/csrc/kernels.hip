@@ -29,12 +29,37 @@ struct KT<double> {
   __device__ static inline double k(int e) { return cK0d[e]; }
 };
 
+// Halo staging tables, the same for every block: entry (spoke << 8) | local
+// index of the block-relative node (x,y,z) a halo position stands for.
+// cHalo6: x,y,z in [-1,4] (6^3, apply); cHalo5: x,y,z in [-1,3] (5^3 cells,
+// apply); cHaloC[c]: x,y,z in [p-1, p+3] around the nodes of colour c.
+__constant__ int cHalo6[216];
+__constant__ int cHalo5[125];
+__constant__ int cHaloC[8 * 125];
+
+static int host_entry(int x, int y, int z) {
+  auto side = [](int v) { return v < 0 ? 0 : (v > 3 ? 2 : 1); };
+  const int sp = side(x) * 9 + side(y) * 3 + side(z);
+  const int q = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+  return (sp << 8) | q;
+}
+
 void upload_k0(const double* K0, hipStream_t st) {
   float f[576];
   for (int i = 0; i < 576; ++i) f[i] = (float)K0[i];
   (void)st;
   hipMemcpyToSymbol(HIP_SYMBOL(cK0d), K0, sizeof(double) * 576, 0, hipMemcpyHostToDevice);
   hipMemcpyToSymbol(HIP_SYMBOL(cK0f), f, sizeof(float) * 576, 0, hipMemcpyHostToDevice);
+  int h6[216], h5[125], hc[8 * 125];
+  for (int h = 0; h < 216; ++h) h6[h] = host_entry(h / 36 - 1, (h / 6) % 6 - 1, h % 6 - 1);
+  for (int h = 0; h < 125; ++h) h5[h] = host_entry(h / 25 - 1, (h / 5) % 5 - 1, h % 5 - 1);
+  for (int c = 0; c < 8; ++c) {
+    const int px = (c >> 2) & 1, py = (c >> 1) & 1, pz = c & 1;
+    for (int h = 0; h < 125; ++h) hc[c * 125 + h] = host_entry(px - 1 + h / 25, py - 1 + (h / 5) % 5, pz - 1 + h % 5);
+  }
+  hipMemcpyToSymbol(HIP_SYMBOL(cHalo6), h6, sizeof(h6), 0, hipMemcpyHostToDevice);
+  hipMemcpyToSymbol(HIP_SYMBOL(cHalo5), h5, sizeof(h5), 0, hipMemcpyHostToDevice);
+  hipMemcpyToSymbol(HIP_SYMBOL(cHaloC), hc, sizeof(hc), 0, hipMemcpyHostToDevice);
 }
 
 // fused multiply-add in the working precision; row sums are written as FMA
@@ -173,13 +198,16 @@ __global__ void __launch_bounds__(TPB) k_apply(int nb, const int* __restrict__ n
 // from LDS.  All lanes visit the cells I = 0..7 in step, so K0[I][J] are
 // wave-uniform constants.
 static constexpr int AP_WAVES = 4;
-template <class T, int MODE>
+// T: storage type of the vectors and moduli; A: arithmetic type of the row
+// sums (FP32 vectors may be summed in FP64, which gfx950 issues at the same
+// rate as FP32 on the vector ALUs)
+template <class T, class A, int MODE>
 __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* __restrict__ nbr,
                                                              const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ E, T dx, const T* __restrict__ in,
                                                              const T* __restrict__ b, T* __restrict__ out) {
-  __shared__ T hu[AP_WAVES][3][216];
-  __shared__ T he[AP_WAVES][128];
+  __shared__ A hu[AP_WAVES][3][216];
+  __shared__ A he[AP_WAVES][128];
   __shared__ int ht[AP_WAVES][28];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * AP_WAVES + w;
@@ -187,53 +215,53 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
   for (int h = lane; h < 216; h += 64) {
-    const int x = h / 36 - 1, y = (h / 6) % 6 - 1, z = h % 6 - 1;
-    const int bb = ht[w][ring_spoke(x, y, z)];
-    const int q = loc3(x, y, z);
+    const int t = cHalo6[h];
+    const int bb = ht[w][t >> 8];
+    const int q = t & 255;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? in[vidx(bb, c, q)] : T(0);
+    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? (A)in[vidx(bb, c, q)] : A(0);
   }
   for (int h = lane; h < 125; h += 64) {
-    const int x = h / 25 - 1, y = (h / 5) % 5 - 1, z = h % 5 - 1;
-    const int bb = ht[w][ring_spoke(x, y, z)];
-    he[w][h] = bb >= 0 ? E[bb * 64 + loc3(x, y, z)] : T(0);
+    const int t = cHalo5[h];
+    const int bb = ht[w][t >> 8];
+    he[w][h] = bb >= 0 ? (A)E[bb * 64 + (t & 255)] : A(0);
   }
   __syncthreads();
   const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
-  T uu[27][3];
+  A uu[27][3];
 #pragma unroll
   for (int s = 0; s < 27; ++s) {
     const int h = (li + s / 9) * 36 + (lj + (s / 3) % 3) * 6 + (lk + s % 3);
 #pragma unroll
     for (int c = 0; c < 3; ++c) uu[s][c] = hu[w][c][h];
   }
-  T f0 = 0, f1 = 0, f2 = 0;
+  A f0 = 0, f1 = 0, f2 = 0;
 #pragma unroll
   for (int I = 0; I < 8; ++I) {
     const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
-    const T e = he[w][(li - ix + 1) * 25 + (lj - iy + 1) * 5 + (lk - iz + 1)];
-    T t0 = 0, t1 = 0, t2 = 0;
+    const A e = he[w][(li - ix + 1) * 25 + (lj - iy + 1) * 5 + (lk - iz + 1)];
+    A t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
     for (int J = 0; J < 8; ++J) {
       const int jx = (J >> 2) & 1, jy = (J >> 1) & 1, jz = J & 1;
       const int sp = (jx - ix + 1) * 9 + (jy - iy + 1) * 3 + (jz - iz + 1);
       const int kb = (I * 8 + J) * 9;
-      t0 = fmaT(KT<T>::k(kb + 0), uu[sp][0], t0);
-      t0 = fmaT(KT<T>::k(kb + 1), uu[sp][1], t0);
-      t0 = fmaT(KT<T>::k(kb + 2), uu[sp][2], t0);
-      t1 = fmaT(KT<T>::k(kb + 3), uu[sp][0], t1);
-      t1 = fmaT(KT<T>::k(kb + 4), uu[sp][1], t1);
-      t1 = fmaT(KT<T>::k(kb + 5), uu[sp][2], t1);
-      t2 = fmaT(KT<T>::k(kb + 6), uu[sp][0], t2);
-      t2 = fmaT(KT<T>::k(kb + 7), uu[sp][1], t2);
-      t2 = fmaT(KT<T>::k(kb + 8), uu[sp][2], t2);
+      t0 = fmaT(KT<A>::k(kb + 0), uu[sp][0], t0);
+      t0 = fmaT(KT<A>::k(kb + 1), uu[sp][1], t0);
+      t0 = fmaT(KT<A>::k(kb + 2), uu[sp][2], t0);
+      t1 = fmaT(KT<A>::k(kb + 3), uu[sp][0], t1);
+      t1 = fmaT(KT<A>::k(kb + 4), uu[sp][1], t1);
+      t1 = fmaT(KT<A>::k(kb + 5), uu[sp][2], t1);
+      t2 = fmaT(KT<A>::k(kb + 6), uu[sp][0], t2);
+      t2 = fmaT(KT<A>::k(kb + 7), uu[sp][1], t2);
+      t2 = fmaT(KT<A>::k(kb + 8), uu[sp][2], t2);
     }
     f0 = fmaT(e, t0, f0);
     f1 = fmaT(e, t1, f1);
     f2 = fmaT(e, t2, f2);
   }
   if (!live) return;
-  const T f[3] = {dx * f0, dx * f1, dx * f2};
+  const T f[3] = {(T)((A)dx * f0), (T)((A)dx * f1), (T)((A)dx * f2)};
   if (MODE == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[vidx(blk, c, lane)] = f[c];
@@ -393,9 +421,9 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_gs0_colour(int nb, int colour
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
   for (int h = lane; h < 125; h += 64) {
-    const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
-    const int bb = ht[w][ring_spoke(x, y, z)];
-    const int q = loc3(x, y, z);
+    const int t = cHaloC[colour * 125 + h];
+    const int bb = ht[w][t >> 8];
+    const int q = t & 255;
 #pragma unroll
     for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
   }
@@ -449,9 +477,9 @@ __global__ void __launch_bounds__(512) k_gs0_colour_uc(int nb, int colour, const
   __syncthreads();
   for (int h = tid; h < UC_BLOCKS * 125; h += 512) {
     const int bl = h / 125, hh = h - bl * 125;
-    const int x = px - 1 + hh / 25, y = py - 1 + (hh / 5) % 5, z = pz - 1 + hh % 5;
-    const int bb = ht[bl][ring_spoke(x, y, z)];
-    const int q = loc3(x, y, z);
+    const int t = cHaloC[colour * 125 + hh];
+    const int bb = ht[bl][t >> 8];
+    const int q = t & 255;
 #pragma unroll
     for (int c = 0; c < 3; ++c) hu[c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
   }
@@ -546,9 +574,9 @@ __global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const i
   __syncthreads();
   for (int h = tid; h < Q_BLOCKS * 125; h += 256) {
     const int bl = h / 125, hh = h - bl * 125;
-    const int x = px - 1 + hh / 25, y = py - 1 + (hh / 5) % 5, z = pz - 1 + hh % 5;
-    const int bb = ht[bl][ring_spoke(x, y, z)];
-    const int q = loc3(x, y, z);
+    const int t = cHaloC[colour * 125 + hh];
+    const int bb = ht[bl][t >> 8];
+    const int q = t & 255;
 #pragma unroll
     for (int c = 0; c < 3; ++c) hu[bl][c][hh] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
   }
@@ -726,9 +754,9 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
   for (int h = lane; h < 125; h += 64) {
-    const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
-    const int bb = ht[w][ring_spoke(x, y, z)];
-    const int q = loc3(x, y, z);
+    const int t = cHaloC[colour * 125 + h];
+    const int bb = ht[w][t >> 8];
+    const int q = t & 255;
 #pragma unroll
     for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
   }
@@ -1774,15 +1802,32 @@ __global__ void k_set_entries(double* v, const int* idx, const double* val, int6
 }
 
 // ======================================================== launch wrappers
-template <class T>
-void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {
+// The FP32 operator of the Krylov recurrence (apply mode 2, q = P K s) sums
+// its rows in FP64 with FP32 in / out: FP32 accumulation of the 576-term rows
+// loses accuracy to cancellation on smooth search directions and cost the
+// FP32 MG-PCG 23 instead of 18 iterations on the 256^3 shell (the FP64 solve
+// and the reference's float mode take 18).  The V-cycle residual (mode 1)
+// keeps FP32 sums: the preconditioner tolerates them.  SPG_ACC64_MODES
+// (bit m = apply mode m) overrides the choice.
+static int acc64_modes() {
+  const char* e = std::getenv("SPG_ACC64_MODES");
+  return e ? std::atoi(e) : 4;
+}
+template <class T, class A>
+static void launch_apply_t(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out,
+                           int mode) {
   const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
   if (mode == 0)
-    k_apply_lds<T, 0><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, A, 0><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
   else if (mode == 2)
-    k_apply_lds<T, 2><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, A, 2><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
   else
-    k_apply_lds<T, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+    k_apply_lds<T, A, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+}
+template <class T>
+void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {
+  if (sizeof(T) == 4 && ((acc64_modes() >> mode) & 1)) launch_apply_t<T, double>(st, L, E, dx, in, b, out, mode);
+  else launch_apply_t<T, T>(st, L, E, dx, in, b, out, mode);
 }
 static ColourOrder colour_order(bool level0, bool forward) {
   ColourOrder o{0};

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -65,6 +66,8 @@ template <> const double* Solver::E<double>() const { return E64_.as<double>(); 
 
 Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   const double t0 = now_s();
+  if (const char* e = std::getenv("SPG_BOTTOM_FP64")) bottom64_ = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SPG_COARSE_FP64")) coarse64_ = std::atoi(e) != 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw SpgError(SPG_E_HIP, "no HIP device available");
   if (cfg.device < 0 || cfg.device >= ndev) throw SpgError(SPG_E_INVALID, "device ordinal out of range");
@@ -369,14 +372,75 @@ void Solver::bottom(const T* b, T* u) {
   T* x = f ? (T*)xb32_.p : (T*)xb64_.p;
   T* y = f ? (T*)yb32_.p : (T*)yb64_.p;
   launch_bottom_gather<T>(st_, R.d, var_of_.as<int>(), b, x);
-  launch_symv<T>(st_, nT_, f ? (const T*)tiles32_.p : (const T*)tiles64_.p, x, f ? (T*)pb32_.p : (T*)pb64_.p, y);
+  if (f && bottom64_) {
+    // FP32 cycle with the FP64 inverse: widen, apply, round back
+    const int64_t npad = (int64_t)nT_ * 64;
+    launch_cast_f2d(st_, (const float*)x, xb64_.as<double>(), npad);
+    launch_symv<double>(st_, nT_, tiles64_.as<double>(), xb64_.as<double>(), pb64_.as<double>(), yb64_.as<double>());
+    launch_cast_d2f(st_, yb64_.as<double>(), (float*)y, npad);
+  } else {
+    launch_symv<T>(st_, nT_, f ? (const T*)tiles32_.p : (const T*)tiles64_.p, x, f ? (T*)pb32_.p : (T*)pb64_.p, y);
+  }
   launch_bottom_scatter<T>(st_, R.d, var_of_.as<int>(), y, b, u);
+}
+
+// FP32 cycle whose explicit levels (and bottom) run in FP64: the level
+// handed over is restricted in FP32 and widened, and widened corrections are
+// rounded back before the FP32 prolongation
+void Solver::vcycle_coarse64(const float* b0, float* z) {
+  const int L = lay_.nlev;
+  const int ex = lay_.explicit_level;
+  launch_zero<float>(st_, z, lv_[0].n3);
+  for (int l = 0; l < L - 1; ++l) {
+    const bool d = l >= ex, dn = l + 1 >= ex;
+    const bool sm = (l == 0 || l >= ex);
+    if (d) {
+      double* u = lv_[l].u<double>();
+      const double* b = lv_[l].b<double>();
+      smooth<double>(l, u, b, true);
+      launch_zero<double>(st_, lv_[l + 1].u<double>(), lv_[l + 1].n3);
+      level_residual<double>(l, u, b, lv_[l].r<double>());
+      launch_restrict<double>(st_, lv_[l + 1].d, lv_[l].r<double>(), lv_[l + 1].b<double>());
+    } else {
+      float* u = l == 0 ? z : lv_[l].u<float>();
+      const float* b = l == 0 ? b0 : lv_[l].b<float>();
+      if (sm) smooth<float>(l, u, b, true);
+      if (dn) launch_zero<double>(st_, lv_[l + 1].u<double>(), lv_[l + 1].n3);
+      else launch_zero<float>(st_, lv_[l + 1].u<float>(), lv_[l + 1].n3);
+      if (sm) {
+        level_residual<float>(l, u, b, lv_[l].r<float>());
+        launch_restrict<float>(st_, lv_[l + 1].d, lv_[l].r<float>(), lv_[l + 1].b<float>());
+      } else {
+        launch_restrict<float>(st_, lv_[l + 1].d, b, lv_[l + 1].b<float>());
+      }
+      if (dn) launch_cast_f2d(st_, lv_[l + 1].b<float>(), lv_[l + 1].b<double>(), lv_[l + 1].n3);
+    }
+  }
+  if (L - 1 >= ex) bottom<double>(lv_[L - 1].b<double>(), lv_[L - 1].u<double>());
+  else bottom<float>(lv_[L - 1].b<float>(), lv_[L - 1].u<float>());
+  for (int l = L - 2; l >= 0; --l) {
+    const bool d = l >= ex, dn = l + 1 >= ex;
+    if (d) {
+      launch_prolongate<double>(st_, lv_[l].d, lv_[l + 1].u<double>(), lv_[l].u<double>());
+      smooth<double>(l, lv_[l].u<double>(), lv_[l].b<double>(), false);
+    } else {
+      float* u = l == 0 ? z : lv_[l].u<float>();
+      const float* b = l == 0 ? b0 : lv_[l].b<float>();
+      if (dn) launch_cast_d2f(st_, lv_[l + 1].u<double>(), lv_[l + 1].u<float>(), lv_[l + 1].n3);
+      launch_prolongate<float>(st_, lv_[l].d, lv_[l + 1].u<float>(), u);
+      if (l == 0 || l >= ex) smooth<float>(l, u, b, false);
+    }
+  }
 }
 
 template <class T>
 void Solver::vcycle(const T* b0, T* z) {
   const int L = lay_.nlev;
   const int ex = lay_.explicit_level;
+  if (sizeof(T) == 4 && coarse64_ && lay_.use_mg && L > ex) {
+    vcycle_coarse64(reinterpret_cast<const float*>(b0), reinterpret_cast<float*>(z));
+    return;
+  }
   if (!lay_.use_mg) {
     launch_copy<T>(st_, b0, z, lv_[0].n3);
     return;

@@ -85,6 +85,7 @@ class Solver {
 
  private:
   template <class T> void vcycle(const T* b0, T* z);
+  void vcycle_coarse64(const float* b0, float* z);
   template <class T> void smooth(int l, T* u, const T* b, bool forward);
   template <class T> void level_residual(int l, const T* u, const T* b, T* r);
   template <class T> void bottom(const T* b, T* u);
@@ -110,6 +111,8 @@ class Solver {
   // bottom
   int64_t nvar_ = 0;
   int nT_ = 0;  // 64-row tiles of the bottom inverse
+  bool bottom64_ = false;  // FP32 cycles apply the FP64 bottom inverse
+  bool coarse64_ = false;  // FP32 cycles run the explicit levels in FP64
   DBuf var_of_, info_, tiles64_, tiles32_, xb64_, yb64_, xb32_, yb32_, pb64_, pb32_;
   // PCG / outer vectors (level 0)
   DBuf x32_, rr32_, s32_, q32_, z32_, x64_, rr64_, s64_, q64_, z64_;
