@@ -94,6 +94,9 @@ def main():
     nn = info["active_nodes"]
     t_ap32 = s.bench_apply(0, args.apply_reps)
     t_ap64 = s.bench_apply(1, args.apply_reps)
+    t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
+    t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
+    log(f"[bench] V-cycle fp32 {t_vc32 * 1e3:.3f} ms, fp64 {t_vc64 * 1e3:.3f} ms")
     gbs32 = BYTES_PER_NODE_FP32 * nn / t_ap32 / 1e9
     gbs64 = BYTES_PER_NODE_FP64 * nn / t_ap64 / 1e9
     log(f"[bench] apply fp32 {t_ap32 * 1e6:.1f} us ({3 * nn / t_ap32 / 1e6:.0f} MDOF/s, {gbs32:.0f} GB/s); "
@@ -135,6 +138,10 @@ def main():
         "hbm_footprint_gb": info["hbm_bytes"] / 1e9,
         "apply_mdof_s_fp32": 3 * nn / t_ap32 / 1e6,
         "apply_mdof_s_fp64": 3 * nn / t_ap64 / 1e6,
+        "apply_us_fp32": t_ap32 * 1e6,
+        "apply_us_fp64": t_ap64 * 1e6,
+        "vcycle_ms_fp32": t_vc32 * 1e3,
+        "vcycle_ms_fp64": t_vc64 * 1e3,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_apply<float> (level-0 stiffness apply)",
@@ -142,7 +149,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_apply_lds<float, 2>", nn),
+            "traffic": measured_traffic("k_apply_lds<float, double, 2>", nn),
         },
         "cpu_baseline": cpu,
     }
