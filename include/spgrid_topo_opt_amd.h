@@ -153,7 +153,8 @@ int spg_level_nodes(spg_solver *s, int32_t level, int32_t *xyz /*host [nb*64*3]*
 #define SPG_OP_VCYCLE 7       /* out = V-cycle(b) (level 0)                    */
 #define SPG_OP_BOTTOM 8       /* out = A^-1 b on the bottom level              */
 /* Run one operator on host vectors (copied to/from the device);
- * precision 0 = FP32 kernels, 1 = FP64 kernels; `forward` for GS order.     */
+ * precision 0 = FP32 kernels, 1 = FP64 kernels; bit 0 of `forward` selects
+ * the GS colour order, bits 8+ the number of boundary sweeps (0 = one).    */
 int spg_kernel(spg_solver *s, int32_t op, int32_t level, int32_t precision, int32_t forward, const double *in,
                const double *b, double *out);
 /* FP64 Galerkin stencils of an explicit level: [nb*64*243] host, row-major

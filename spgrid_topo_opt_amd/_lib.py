@@ -266,12 +266,14 @@ class Solver:
         _check(lib().spg_level_nodes(self.h, level, _ip(xyz), fl.ctypes.data_as(C.POINTER(C.c_uint8))))
         return xyz, fl
 
-    def kernel(self, op, level, inp=None, b=None, precision=1, forward=True, out_level=None):
+    def kernel(self, op, level, inp=None, b=None, precision=1, forward=True, out_level=None, sweeps=1):
+        """one operator on host vectors; `sweeps` boundary sweeps for OP_GS_BOUNDARY"""
         nout = self.level_blocks(level if out_level is None else out_level) * 64
         out = np.zeros((nout, 3))
         inp = None if inp is None else np.ascontiguousarray(inp, np.float64)
         b = None if b is None else np.ascontiguousarray(b, np.float64)
-        _check(lib().spg_kernel(self.h, op, level, precision, int(forward), _dp(inp), _dp(b), _dp(out)))
+        arg = int(bool(forward)) | (int(sweeps) << 8)
+        _check(lib().spg_kernel(self.h, op, level, precision, arg, _dp(inp), _dp(b), _dp(out)))
         return out
 
     def stencils(self, level):

@@ -994,7 +994,7 @@ __device__ __forceinline__ void bnd_fetch(BndFetch<T>& F, int i, bool live, int 
   F.j[1] = jj.y;
   F.j[2] = jj.z;
   F.j[3] = jj.w;
-  if (g == 5) {
+  if (g == 5 && bt) {
     const V* R = reinterpret_cast<const V*>(bt + (size_t)i * 4);
 #pragma unroll
     for (int q = 0; q < 4 / PER; ++q) {
@@ -1079,7 +1079,12 @@ __device__ __forceinline__ BndCursor bnd_advance(const BndCursor& a, const BndOf
   return b;
 }
 
-template <class T, int NT>
+// MAP = false: one boundary phase on (u, b~).  MAP = true: workgroup j
+// runs the phase on a unit vector instead and writes column j of the dense
+// map [G H] of the phase, u_B <- G u_B + H b~ (row-major, 3n x 6n, entry
+// index node*3 + component): columns j < 3n start from u_B = e_j with
+// b~ = 0, columns 3n + j from u_B = 0 with b~ = e_j.
+template <class T, int NT, bool MAP = false>
 __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder ord, int sweeps,
                                                    const int* __restrict__ list, const int* __restrict__ bci,
                                                    const T* __restrict__ A, const uint8_t* __restrict__ flags,
@@ -1088,16 +1093,21 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* ub = reinterpret_cast<T*>(smem_raw);                 // [3][n]
   uint8_t* fb = reinterpret_cast<uint8_t*>(ub + 3 * n);  // [n]
+  const int col = MAP ? (int)blockIdx.x : 0;
+  const bool hcol = MAP && col >= 3 * n;
+  const int jn = (hcol ? col - 3 * n : col) / 3, jc = (hcol ? col - 3 * n : col) % 3;
   for (int i = threadIdx.x; i < n; i += NT) {
     const int node = list[i], blk = node >> 6, l = node & 63;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ub[c * n + i] = u[vidx(blk, c, l)];
+    for (int c = 0; c < 3; ++c)
+      ub[c * n + i] = MAP ? ((!hcol && i == jn && c == jc) ? T(1) : T(0)) : u[vidx(blk, c, l)];
     fb[i] = flags[node];
   }
   const int oct = threadIdx.x >> 3, g = threadIdx.x & 7;
   auto fetch = [&](BndFetch<T>& F, const BndCursor& k) {
     const int i = L.off[k.c] + k.p * NO + oct;
-    bnd_fetch<T>(F, i, i < L.off[k.c + 1], g, bci, A, bt);
+    bnd_fetch<T>(F, i, i < L.off[k.c + 1], g, bci, A, MAP ? nullptr : bt);
+    if (MAP && hcol && g == 5 && i == jn) F.r[jc] = T(1);
   };
   BndCursor cur{0, 0, 0, ord.at(0)};
   BndFetch<T> X, Y;
@@ -1120,11 +1130,49 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
     if (!more) break;
     cur = nx;
   }
+  if (MAP) {
+    const int64_t K = 6 * (int64_t)n;
+    for (int i = threadIdx.x; i < n; i += NT)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) u[(int64_t)(i * 3 + cc) * K + col] = ub[cc * n + i];
+    return;
+  }
   for (int i = threadIdx.x; i < n; i += NT) {
     const int node = list[i], blk = node >> 6, l = node & 63;
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n + i];
   }
+}
+
+// Dense boundary map apply: x = [u_B; b~] (node-major), y = [G H] x, one
+// wavefront per row with a fixed-order lane reduction.
+template <class T>
+__global__ void k_bmap_gather(int n, const int* __restrict__ list, const T* __restrict__ u,
+                              const T* __restrict__ bt, T* __restrict__ x) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int i = t / 3, c = t - 3 * (t / 3), node = list[i];
+  x[t] = u[vidx(node >> 6, c, node & 63)];
+  x[3 * n + t] = bt[i * 4 + c];
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_bmap_gemv(int rows, int K, const T* __restrict__ M, const T* __restrict__ x,
+                                                   T* __restrict__ y) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  const T* row = M + (int64_t)r * K;
+  T acc = 0;
+  for (int k = lane; k < K; k += 64) acc = fmaT(row[k], x[k], acc);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) y[r] = acc;
+}
+template <class T>
+__global__ void k_bmap_scatter(int n, const int* __restrict__ list, const T* __restrict__ y, T* __restrict__ u) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int i = t / 3, c = t - 3 * (t / 3), node = list[i];
+  u[vidx(node >> 6, c, node & 63)] = y[t];
 }
 // ------------------------------------- level-0 matrix-free boundary phase
 // Level 0 needs no coefficient rows: boundary node i carries the moduli of
@@ -1931,6 +1979,21 @@ void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double
   const int n = L.bnd_off[8];
   if (n > 0) k_bnd_coef0<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, L.nbr, E, dx, A);
 }
+void launch_bmap_build(hipStream_t st, const LevelDev& L, int sweeps, bool forward, double* M) {
+  const int n = L.bnd_off[8];
+  if (n == 0) return;
+  const size_t lds = (size_t)3 * n * sizeof(double) + (size_t)n;
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute((const void*)k_bnd_sweeps<double, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kMaxLds);
+    return true;
+  }();
+  (void)attr;
+  BndOff o;
+  for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
+  k_bnd_sweeps<double, 512, true><<<6 * n, 512, lds, st>>>(n, o, colour_order(false, forward), sweeps, L.bnd, L.bci,
+                                                           L.bA64, L.flags, nullptr, M);
+}
 void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A) {
   const int n = L.bnd_off[8];
   if (n > 0) k_bnd_coef_st<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, ST, A);
@@ -1982,9 +2045,32 @@ void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T*
   k_st_colour<T, 0><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, ST, u, b, nullptr);
 }
 template <class T>
+struct BmapPtr;
+template <>
+struct BmapPtr<float> {
+  static const float* M(const LevelDev& L, bool fwd) { return L.bmap32[fwd ? 0 : 1]; }
+  static float* x(const LevelDev& L) { return L.bmx32; }
+};
+template <>
+struct BmapPtr<double> {
+  static const double* M(const LevelDev& L, bool fwd) { return L.bmap64[fwd ? 0 : 1]; }
+  static double* x(const LevelDev& L) { return L.bmx64; }
+};
+template <class T>
 void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int sweeps, bool forward) {
   if (sweeps <= 0 || L.bnd_off[8] == 0) return;
   const int path = boundary_path();
+  const T* M = BmapPtr<T>::M(L, forward);
+  if (path == 0 && M && sweeps == L.bmap_sweeps) {
+    const int n = L.bnd_off[8];
+    T* x = BmapPtr<T>::x(L);  // [6n] input, then [3n] output behind it
+    T* bt = BndPtr<T>::bt(L);
+    k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, BndPtr<T>::A(L), u, b, bt);
+    k_bmap_gather<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, u, bt, x);
+    k_bmap_gemv<T><<<(3 * n + 3) / 4, 256, 0, st>>>(3 * n, 6 * n, M, x, x + 6 * n);
+    k_bmap_scatter<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, x + 6 * n, u);
+    return;
+  }
   if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, false, forward)) return;
   const ColourOrder ord = colour_order(false, forward);
   if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {

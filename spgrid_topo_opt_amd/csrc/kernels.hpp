@@ -37,6 +37,14 @@ struct LevelDev {
   const double* b0E64 = nullptr;
   const float* b0D32 = nullptr;
   const double* b0D64 = nullptr;
+  // explicit levels: dense maps [G H] of one boundary phase (forward,
+  // backward colour order) for `bmap_sweeps` sweeps, and their vector
+  // scratch ([6n] input then [3n] output)
+  const float* bmap32[2] = {nullptr, nullptr};
+  const double* bmap64[2] = {nullptr, nullptr};
+  float* bmx32 = nullptr;
+  double* bmx64 = nullptr;
+  int bmap_sweeps = 0;
 };
 
 // Colour of the i-th step of a Gauss-Seidel sweep: level 0 runs the colours
@@ -129,6 +137,8 @@ void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double
 void launch_bnd0_setup(hipStream_t st, const LevelDev& L, const int* cslot, const double* E, double dx, double* bE,
                        double* bD);
 void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A);
+// dense FP64 map [G H] (3n x 6n, row-major) of a boundary phase on an explicit level
+void launch_bmap_build(hipStream_t st, const LevelDev& L, int sweeps, bool forward, double* M);
 // bottom
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
                            double* A);

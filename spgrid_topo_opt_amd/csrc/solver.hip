@@ -30,6 +30,9 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// boundary phases of explicit levels with at most this many unknowns run as
+// one dense map (2 maps x 3n x 6n entries in FP64 and FP32)
+static constexpr int kMaxDenseBoundary = 6144;
 // relative residual reduction one FP32 MG-PCG pass is asked for at most
 static constexpr double kFp32PassReduction = 1e-6;
 // a bottom level above this many unknowns is refused (dense FP64 inverse)
@@ -278,6 +281,27 @@ void Solver::build_boundary_rows() {
     else if (lay_.lv[l].is_explicit) launch_bnd_coef_st(st_, R.d, R.st64.as<double>(), R.bA64.as<double>());
     else continue;
     launch_cast_d2f(st_, R.bA64.as<double>(), R.bA32.as<float>(), (int64_t)n * 288);
+    // small boundary sets of explicit levels: each phase becomes one dense map
+    if (l > 0 && lay_.lv[l].is_explicit && 3 * n <= kMaxDenseBoundary && cfg_.boundary_iters > 0) {
+      const size_t entries = (size_t)3 * n * 6 * n;
+      for (int d = 0; d < 2; ++d) {
+        if (!R.bmap64[d].p) {
+          R.bmap64[d].alloc(entries * 8);
+          R.bmap32[d].alloc(entries * 4);
+        }
+        launch_bmap_build(st_, R.d, cfg_.boundary_iters, d == 0, R.bmap64[d].as<double>());
+        launch_cast_d2f(st_, R.bmap64[d].as<double>(), R.bmap32[d].as<float>(), (int64_t)entries);
+        R.d.bmap64[d] = R.bmap64[d].as<double>();
+        R.d.bmap32[d] = R.bmap32[d].as<float>();
+      }
+      if (!R.bmx64.p) {
+        R.bmx64.alloc((size_t)9 * n * 8);
+        R.bmx32.alloc((size_t)9 * n * 4);
+      }
+      R.d.bmx64 = R.bmx64.as<double>();
+      R.d.bmx32 = R.bmx32.as<float>();
+      R.d.bmap_sweeps = cfg_.boundary_iters;
+    }
   }
   sync();
 }
@@ -742,15 +766,17 @@ void Solver::run_kernel(int op, int level, int precision, int forward, const dou
         put(A, in, nb);
         put(Bv, b, nb);
         const LevelDev& Ld = lv_[level].d;
+        const bool fwd = (forward & 1) != 0;
+        const int sweeps = std::max(1, forward >> 8);
         if (op == SPG_OP_GS_BOUNDARY) {
-          if (level == 0) launch_gs0_boundary<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), 1, forward != 0);
-          else launch_st_boundary<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), 1, forward != 0);
+          if (level == 0) launch_gs0_boundary<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), sweeps, fwd);
+          else launch_st_boundary<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), sweeps, fwd);
         } else {
           for (int c = 0; c < 8; ++c) {
             if (level == 0)
-              launch_gs0<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(true, forward != 0, c));
+              launch_gs0<T>(st_, Ld, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(true, fwd, c));
             else
-              launch_st_gs<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(false, forward != 0, c));
+              launch_st_gs<T>(st_, Ld, lv_[level].st<T>(), A.as<T>(), Bv.as<T>(), sweep_colour(false, fwd, c));
           }
         }
         get(A.as<T>(), nb);

@@ -150,6 +150,31 @@ def test_boundary_fallback_paths(blocky, path, monkeypatch):
             assert rel(got, want) < tol, (lv, prec)
 
 
+@pytest.mark.parametrize("path", ["0", "1"])
+def test_boundary_phase_multi_sweep(blocky, path, monkeypatch):
+    """a whole boundary phase (as many sweeps as the V-cycle runs) on every
+    smoothed level; on the explicit levels path 0 runs it as one dense map
+    [G H] built from the phase on unit vectors, path 1 sweep by sweep"""
+    P = blocky
+    monkeypatch.setenv("SPG_BOUNDARY_PATH", path)
+    k = P.p.boundary_iters
+    rng = np.random.default_rng(12)
+    for lv in [0] + list(range(2, P.o.levels)):
+        n = len(P.map[lv][0])
+        gf = P.map[lv][1]
+        u = rng.standard_normal((n, 3)) * 1e-3
+        u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+        b = rng.standard_normal((n, 3))
+        for fwd in (True, False):
+            want = P.to_o(lv, u)
+            for _ in range(k):
+                want = P.o.gs_sweep(lv, want, P.to_o(lv, b), fwd, True)
+            want = P.from_o(lv, want)
+            for prec, tol in ((1, 1e-11), (0, 1e-4)):
+                got = P.g.kernel(_lib.OP_GS_BOUNDARY, lv, u, b, precision=prec, forward=fwd, sweeps=k)
+                assert rel(got, want) < tol, (lv, fwd, prec)
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 1e-6)])
 def test_restriction_and_prolongation(blocky, prec, tol):
     P = blocky
