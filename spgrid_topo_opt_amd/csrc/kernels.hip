@@ -1079,12 +1079,7 @@ __device__ __forceinline__ BndCursor bnd_advance(const BndCursor& a, const BndOf
   return b;
 }
 
-// MAP = false: one boundary phase on (u, b~).  MAP = true: workgroup j
-// runs the phase on a unit vector instead and writes column j of the dense
-// map [G H] of the phase, u_B <- G u_B + H b~ (row-major, 3n x 6n, entry
-// index node*3 + component): columns j < 3n start from u_B = e_j with
-// b~ = 0, columns 3n + j from u_B = 0 with b~ = e_j.
-template <class T, int NT, bool MAP = false>
+template <class T, int NT>
 __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder ord, int sweeps,
                                                    const int* __restrict__ list, const int* __restrict__ bci,
                                                    const T* __restrict__ A, const uint8_t* __restrict__ flags,
@@ -1093,21 +1088,16 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* ub = reinterpret_cast<T*>(smem_raw);                 // [3][n]
   uint8_t* fb = reinterpret_cast<uint8_t*>(ub + 3 * n);  // [n]
-  const int col = MAP ? (int)blockIdx.x : 0;
-  const bool hcol = MAP && col >= 3 * n;
-  const int jn = (hcol ? col - 3 * n : col) / 3, jc = (hcol ? col - 3 * n : col) % 3;
   for (int i = threadIdx.x; i < n; i += NT) {
     const int node = list[i], blk = node >> 6, l = node & 63;
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      ub[c * n + i] = MAP ? ((!hcol && i == jn && c == jc) ? T(1) : T(0)) : u[vidx(blk, c, l)];
+    for (int c = 0; c < 3; ++c) ub[c * n + i] = u[vidx(blk, c, l)];
     fb[i] = flags[node];
   }
   const int oct = threadIdx.x >> 3, g = threadIdx.x & 7;
   auto fetch = [&](BndFetch<T>& F, const BndCursor& k) {
     const int i = L.off[k.c] + k.p * NO + oct;
-    bnd_fetch<T>(F, i, i < L.off[k.c + 1], g, bci, A, MAP ? nullptr : bt);
-    if (MAP && hcol && g == 5 && i == jn) F.r[jc] = T(1);
+    bnd_fetch<T>(F, i, i < L.off[k.c + 1], g, bci, A, bt);
   };
   BndCursor cur{0, 0, 0, ord.at(0)};
   BndFetch<T> X, Y;
@@ -1130,18 +1120,81 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
     if (!more) break;
     cur = nx;
   }
-  if (MAP) {
-    const int64_t K = 6 * (int64_t)n;
-    for (int i = threadIdx.x; i < n; i += NT)
-#pragma unroll
-      for (int cc = 0; cc < 3; ++cc) u[(int64_t)(i * 3 + cc) * K + col] = ub[cc * n + i];
-    return;
-  }
   for (int i = threadIdx.x; i < n; i += NT) {
     const int node = list[i], blk = node >> 6, l = node & 63;
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n + i];
   }
+}
+
+// Dense map [G H] of a boundary phase, u_B <- G u_B + H b~ (row-major,
+// 3n x 6n, entry index node*3 + component).  Column-parallel: the phase is
+// run on all unit vectors at once, lane = column (columns j < 3n start from
+// u_B = e_j with b~ = 0, columns 3n + j from u_B = 0 with b~ = e_j), so
+// every node row and its coefficients are wave-uniform.  The 16 waves of a
+// workgroup share 64 columns and split the nodes of each colour; the
+// matrix itself holds the iterate (nodes of one colour do not couple).
+constexpr int BM_WAVES = 16;
+__global__ void __launch_bounds__(64 * BM_WAVES) k_bmap_build(int n, BndOff L, ColourOrder ord, int sweeps,
+                                                            const int* __restrict__ list,
+                                                            const int* __restrict__ bci,
+                                                            const double* __restrict__ A,
+                                                            const uint8_t* __restrict__ flags,
+                                                            double* __restrict__ M) {
+  const int64_t K = 6 * (int64_t)n;
+  const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool live = col < K;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int rows = 3 * n;
+  double* Mc = M + col;
+  if (live)
+    for (int r = w; r < rows; r += BM_WAVES) Mc[(int64_t)r * K] = (col == r) ? 1.0 : 0.0;
+  const int hj = col - rows;  // right-hand-side unit index of an H column
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int k = 0; k < 8; ++k) {
+      __syncthreads();
+      const int c = ord.at(k);
+      for (int i = L.off[c] + w; i < L.off[c + 1]; i += BM_WAVES) {
+        const uint8_t fl = flags[list[i]];
+        double* Ui = Mc + (int64_t)(3 * i) * K;
+        if (!live) continue;
+        if (fl & 1) {
+          const double* Ai = A + (size_t)i * 288;
+          const int* Ji = bci + (size_t)i * 32;
+          double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+          for (int g = 0; g < 8; ++g)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int j = Ji[g * 4 + t];
+              if (j < 0) continue;
+              const double* m = Ai + g * 36 + t * 9;
+              const double* Uj = Mc + (int64_t)(3 * j) * K;
+              const double x = Uj[0], y = Uj[K], z = Uj[2 * K];
+              a0 = fmaT(m[0], x, fmaT(m[1], y, fmaT(m[2], z, a0)));
+              a1 = fmaT(m[3], x, fmaT(m[4], y, fmaT(m[5], z, a1)));
+              a2 = fmaT(m[6], x, fmaT(m[7], y, fmaT(m[8], z, a2)));
+            }
+          // self block: spoke 13 = (g 5, t 1)
+          const double* m = Ai + 5 * 36 + 9;
+          double d[6] = {m[0], m[4], m[8], m[1], m[2], m[5]};
+          dirichlet_inverse(d, fl);
+          const double r0 = (hj == 3 * i ? 1.0 : 0.0) - a0;
+          const double r1 = (hj == 3 * i + 1 ? 1.0 : 0.0) - a1;
+          const double r2 = (hj == 3 * i + 2 ? 1.0 : 0.0) - a2;
+          const double u0 = Ui[0] + d[0] * r0 + d[3] * r1 + d[4] * r2;
+          const double u1 = Ui[K] + d[3] * r0 + d[1] * r1 + d[5] * r2;
+          const double u2 = Ui[2 * K] + d[4] * r0 + d[5] * r1 + d[2] * r2;
+          Ui[0] = (fl & 2) ? 0.0 : u0;
+          Ui[K] = (fl & 4) ? 0.0 : u1;
+          Ui[2 * K] = (fl & 8) ? 0.0 : u2;
+        } else {
+          if (fl & 2) Ui[0] = 0.0;
+          if (fl & 4) Ui[K] = 0.0;
+          if (fl & 8) Ui[2 * K] = 0.0;
+        }
+      }
+    }
 }
 
 // Dense boundary map apply: x = [u_B; b~] (node-major), y = [G H] x, one
@@ -1982,17 +2035,10 @@ void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double
 void launch_bmap_build(hipStream_t st, const LevelDev& L, int sweeps, bool forward, double* M) {
   const int n = L.bnd_off[8];
   if (n == 0) return;
-  const size_t lds = (size_t)3 * n * sizeof(double) + (size_t)n;
-  static const bool attr = [] {
-    (void)hipFuncSetAttribute((const void*)k_bnd_sweeps<double, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxLds);
-    return true;
-  }();
-  (void)attr;
   BndOff o;
   for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
-  k_bnd_sweeps<double, 512, true><<<6 * n, 512, lds, st>>>(n, o, colour_order(false, forward), sweeps, L.bnd, L.bci,
-                                                           L.bA64, L.flags, nullptr, M);
+  k_bmap_build<<<(6 * n + 63) / 64, 64 * BM_WAVES, 0, st>>>(n, o, colour_order(false, forward), sweeps, L.bnd, L.bci,
+                                                           L.bA64, L.flags, M);
 }
 void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A) {
   const int n = L.bnd_off[8];
