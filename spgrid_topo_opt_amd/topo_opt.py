@@ -19,10 +19,14 @@ Problem (index block*64 + i*16 + j*4 + k), float64 on cuda.
 """
 from __future__ import annotations
 
+import dataclasses
+import time
+
 import numpy as np
 import torch
 
 from . import _lib
+from . import narrow_band as _nb
 
 
 class FEM:
@@ -47,6 +51,9 @@ class FEM:
         if self._rho_host is None or not np.array_equal(r, self._rho_host):
             self.solver.update_density(r.reshape(-1, 64))
             self._rho_host = r.copy()
+
+    def close(self):
+        self.solver.close()
 
     def solve(self) -> dict:
         _, info = self.solver.solve(want_output=False, **self.kw)
@@ -184,11 +191,13 @@ def filter_sensitivity(grid: CellGrid, dc, rho, filter_size=1.5):
     return torch.where(grid.inside, out, torch.zeros_like(out))
 
 
-def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0):
+def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None):
     """Bisection on the Lagrange multiplier (optimizer/optimality_criterion.cpp:19-146):
     new = clamp(old*sqrt(max(0,-dc)/mid), old-step, old+step), clamped to [min_density, 1];
-    the volume sum over inside cells must equal volume_fraction * #inside."""
-    target = volume_fraction * inside.sum().item()
+    the volume sum over the updated cells (`inside`) must equal volume_fraction
+    times the number of container cells (default: the updated cells)."""
+    n = inside.sum().item() if container_cells is None else container_cells
+    target = volume_fraction * n
     neg = torch.clamp(-dc, min=0.0)
 
     def trial(mid):
@@ -208,11 +217,22 @@ def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0)
 
 
 class TopoOpt:
-    """Minimal SIMP/OC loop on the GPU (reference iterate(): solve, sensitivity,
-    filter, OC update).  The block topology stays fixed."""
+    """SIMP/OC loop on the GPU (reference iterate(), spgrid_topo_opt.cpp:1289-1331):
+    narrow-band update, solve, sensitivity, filter, OC update.
 
-    def __init__(self, prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, **fem_kw):
+    Without `grid_update_start` the block set stays the whole container.  With
+    it, from that iteration on every iteration first updates the thin / fat /
+    fatter block sets (narrow_band.NarrowBand); the solve then runs on the
+    fatter blocks (rebuilt when the fat or fatter set changes), and the
+    gradient clear, filter swap and OC update cover the fat blocks only, as
+    clear_gradients (spgrid_topo_opt.h:869-878), smooth_dc
+    (spgrid_topo_opt.cpp:527-574) and the OC optimizer
+    (optimizer/optimality_criterion.cpp:29) do."""
+
+    def __init__(self, prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, grid_update_start=None,
+                 active_threshold=1e-6, fat_margin=1, connectivity_filtering=True, **fem_kw):
         self.prob = prob
+        self.fem_kw = fem_kw
         self.fem = FEM(prob, **fem_kw)
         self.grid = CellGrid(prob)
         self.volume_fraction = volume_fraction
@@ -222,15 +242,78 @@ class TopoOpt:
         self.rho = torch.where(inside, torch.full_like(inside, volume_fraction, dtype=torch.float64),
                                torch.zeros_like(inside, dtype=torch.float64))
         self.history = []
+        self.iter = 0
+        self.grid_update_start = grid_update_start
+        self.band = None
+        if grid_update_start is not None:
+            ins = (prob.density.reshape(-1) > 0)
+            hf = _nb.force_cells(prob.block_base, ins, prob.force_coord)
+            self.band = _nb.NarrowBand(prob.block_base, ins, hf, minimum_density=prob.min_density,
+                                       active_threshold=active_threshold, fat_margin=fat_margin,
+                                       connectivity_filtering=connectivity_filtering, device=inside.device)
+            self.dc0 = torch.zeros_like(self.rho)
+            self._fem_blocks = torch.arange(self.band.nb, device=inside.device)
+            self._fem_fat = self.band.fat.clone()
+            self._cells = torch.arange(64, device=inside.device)
+
+    def close(self):
+        self.fem.close()
 
     def iterate(self):
+        if self.band is not None:
+            return self._iterate_band()
         rho = self.rho.clone().requires_grad_(True)
         c = compliance(rho, self.fem)
         c.backward()
         dc = filter_sensitivity(self.grid, rho.grad.detach(), self.rho, self.filter_size)
         new, change = oc_update(self.rho, dc, self.grid.inside, self.volume_fraction, self.step_limit)
         self.rho = new
+        self.iter += 1
         rec = {"compliance": c.item(), "change": change, "iterations": self.fem.last_info["iterations"],
                "solve_s": self.fem.last_info["solve_seconds"]}
+        self.history.append(rec)
+        return rec
+
+    def _iterate_band(self):
+        band = self.band
+        dev = self.rho.device
+        t0 = time.perf_counter()
+        if self.iter >= self.grid_update_start:
+            band.update(self.rho)
+        blocks = band.fem_blocks()
+        rebuilt = False
+        if not (torch.equal(blocks, self._fem_blocks) and torch.equal(band.fat, self._fem_fat)):
+            # new topology: a solver for the fatter blocks (fem_solve rebuilds its hierarchy likewise)
+            sub = dataclasses.replace(self.prob, block_base=self.prob.block_base[blocks.cpu().numpy()],
+                                      density=band.fem_density(self.rho, blocks).cpu().numpy())
+            self.fem.close()
+            self.fem = FEM(sub, **self.fem_kw)
+            self._fem_blocks, self._fem_fat = blocks, band.fat.clone()
+            rebuilt = True
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        rho_fem = band.fem_density(self.rho, blocks).reshape(-1).clone().requires_grad_(True)
+        c = compliance(rho_fem, self.fem)
+        c.backward()
+        torch.cuda.synchronize(dev)
+        t2 = time.perf_counter()
+        fat_cells = band.fat.repeat_interleave(64)
+        idx = (blocks[:, None] * 64 + self._cells[None]).reshape(-1)
+        live = (self.grid.inside & fat_cells)[idx]
+        dc0 = torch.where(fat_cells, torch.zeros_like(self.dc0), self.dc0)
+        dc0[idx[live]] = rho_fem.grad.detach()[live]
+        dc1 = filter_sensitivity(self.grid, dc0, self.rho, self.filter_size)
+        self.dc0 = torch.where(fat_cells, dc1, dc0)
+        upd = self.grid.inside & fat_cells
+        new, change = oc_update(self.rho, self.dc0, upd, self.volume_fraction, self.step_limit,
+                                self.prob.min_density, container_cells=self.grid.inside.sum().item())
+        self.rho = new
+        torch.cuda.synchronize(dev)
+        t3 = time.perf_counter()
+        self.iter += 1
+        rec = {"compliance": c.item(), "change": change, "iterations": self.fem.last_info["iterations"],
+               "solve_s": self.fem.last_info["solve_seconds"], "fem_blocks": int(blocks.numel()),
+               "fat_blocks": int(band.fat.sum()), "thin_blocks": int(band.thin.sum()), "rebuilt": rebuilt,
+               "band_s": t1 - t0, "fem_s": t2 - t1, "filter_oc_s": t3 - t2}
         self.history.append(rec)
         return rec

@@ -108,3 +108,34 @@ def test_oc_loop_decreases_compliance():
     vol = opt.rho[inside].mean().item()
     assert abs(vol - 0.3) < 1e-3
     assert recs[-1]["compliance"] < recs[0]["compliance"]
+
+
+def test_narrow_band_loop_follows_the_band_rules():
+    """grid_update_start: each iteration updates the thin / fat / fatter sets
+    (checked against the loop restatement on the same densities), solves on
+    the fatter blocks and keeps the volume target over the whole container"""
+    from oracle.narrow_band import BandOracle
+    from spgrid_topo_opt_amd import narrow_band as NB
+    from spgrid_topo_opt_amd import topo_opt as T
+    p = W.box(32, 16, 8, seed=0, lo=1.0, hi=1.0, mg_levels=3, load="point", name="band_box")
+    opt = T.TopoOpt(p, volume_fraction=0.1, step_limit=0.2, tolerance=1e-6, grid_update_start=1)
+    ins = (p.density.reshape(-1) > 0)
+    o = BandOracle(p.block_base, ins, NB.force_cells(p.block_base, ins, p.force_coord))
+    recs = []
+    for it in range(5):
+        rho = opt.rho.cpu().numpy()
+        recs.append(opt.iterate())
+        if it >= 1:
+            o.update(rho)
+        for name in ("thin", "fat", "fatter"):
+            np.testing.assert_array_equal(getattr(opt.band, name).cpu().numpy(), o.mask(getattr(o, name)),
+                                          err_msg=f"{name} after iteration {it}")
+        assert opt.fem.prob.block_base.shape[0] == o.mask(o.fatter).sum()
+        assert np.isfinite(recs[-1]["compliance"])
+        print(it, recs[-1], flush=True)
+    # the band shrinks and the solve follows it
+    assert recs[-1]["fat_blocks"] < p.block_base.shape[0] and recs[-1]["thin_blocks"] < recs[-1]["fat_blocks"]
+    assert any(r["rebuilt"] for r in recs)
+    inside = opt.grid.inside
+    assert abs(opt.rho[inside].sum().item() / inside.sum().item() - 0.1) < 1e-3
+    opt.close()
