@@ -8,6 +8,12 @@ JSON line on stdout; logs go to stderr.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--levels 5]
                   [--no-cpu] [--cpu-seconds 20]
+
+With --opt-loop it instead times the SIMP/OC optimization loop on the
+512x256x256 band (cells within 24 cells of the bounding-box surface):
+--opt-iters iterations with the narrow band updated from iteration 5, once
+with mixed-precision and once with FP64 solves, and prints seconds per
+optimization step and both final compliances.
 """
 import argparse
 import json
@@ -41,11 +47,20 @@ def main():
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the FP32 apply and a 512 MiB copy (5 launches each) for rocprofv3 --pmc passes")
+    ap.add_argument("--opt-loop", action="store_true", help="time the OC optimization loop on the 512x256x256 band")
+    ap.add_argument("--opt-iters", type=int, default=20)
+    ap.add_argument("--opt-precisions", default="mixed,fp64")
+    ap.add_argument("--opt-band-start", type=int, default=5)
+    ap.add_argument("--opt-tol", type=float, default=1e-4,
+                    help="solve tolerance in the loop (the reference optimizer's cg_tolerance default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is described in DESIGN.md and not implemented.", file=sys.stderr)
         sys.exit(2)
+    if args.opt_loop:
+        opt_loop(args)
+        return
 
     import torch
 
@@ -224,6 +239,72 @@ def cpu_baseline(prob, gpu_iterations, budget_s):
         "setup_s": setup,
         "per_iteration_s": per_it,
     }
+
+
+def opt_loop(args):
+    """SIMP/OC loop (volume fraction 0.1, step 0.2, filter 1.5) on the band
+    workload, narrow band updated every iteration from iteration 5."""
+    import torch
+
+    from spgrid_topo_opt_amd import topo_opt as T
+    from spgrid_topo_opt_amd import workloads as W
+
+    t0 = time.perf_counter()
+    prob = W.config("cfg4")
+    log(f"[opt] problem {prob.name}: {prob.block_base.shape[0]} blocks, {prob.n_active_cells} cells, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    runs = {}
+    for name in args.opt_precisions.split(","):
+        prec = {"mixed": 0, "fp64": 1}[name]
+        t0 = time.perf_counter()
+        opt = T.TopoOpt(prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, grid_update_start=args.opt_band_start,
+                        tolerance=args.opt_tol, precision=prec)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        log(f"[opt] {name}: setup {setup:.2f}s")
+        recs = []
+        t0 = time.perf_counter()
+        for it in range(args.opt_iters):
+            ti = time.perf_counter()
+            r = opt.iterate()
+            r["step_s"] = time.perf_counter() - ti
+            recs.append(r)
+            log(f"[opt] {name} it {it}: c={r['compliance']:.6e} step {r['step_s']:.2f}s solve {r['solve_s']:.3f}s "
+                f"pcg {r['iterations']} fem {r['fem_s']:.2f}s band {r['band_s']:.2f}s filter+oc "
+                f"{r['filter_oc_s']:.2f}s blocks {r['fem_blocks']} fat {r['fat_blocks']} rebuilt {r['rebuilt']}")
+        total = time.perf_counter() - t0
+        runs[name] = {
+            "s_per_step": total / args.opt_iters,
+            "setup_s": setup,
+            "final_compliance": recs[-1]["compliance"],
+            "solve_s_mean": sum(r["solve_s"] for r in recs) / len(recs),
+            "pcg_iterations": [r["iterations"] for r in recs],
+            "rebuilds": sum(bool(r["rebuilt"]) for r in recs),
+            "final_fem_blocks": recs[-1]["fem_blocks"],
+            "compliance_history": [r["compliance"] for r in recs],
+        }
+        opt.close()
+        del opt
+        torch.cuda.empty_cache()
+    first = next(iter(runs.values()))
+    out = {
+        "metric": "seconds per optimization step (solve + sensitivity + filter + OC + band update)",
+        "value": first["s_per_step"],
+        "unit": "s/step",
+        "n_gpus": 1,
+        "steps": args.opt_iters,
+        "higher_is_better": False,
+        "data": "synthetic",
+        "config": {"workload": "band 512x256x256, cells within 24 cells of the bounding-box surface, "
+                               "V=0.1, step 0.2, filter 1.5, 6 MG levels",
+                   "band_update_from_iteration": args.opt_band_start,
+                   "active_cells": prob.n_active_cells, "tolerance": args.opt_tol},
+        "runs": runs,
+    }
+    if "mixed" in runs and "fp64" in runs:
+        a, b = runs["mixed"]["final_compliance"], runs["fp64"]["final_compliance"]
+        out["compliance_rel_diff_mixed_vs_fp64"] = abs(a - b) / abs(b)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

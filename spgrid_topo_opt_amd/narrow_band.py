@@ -181,16 +181,30 @@ class NarrowBand:
         return torch.where(live, d.clamp(min=np.finfo(np.float64).tiny), torch.zeros_like(d))
 
 
+def _block_keys(bc):
+    bc = np.asarray(bc, np.int64)
+    return (bc[..., 0] << 42) + (bc[..., 1] << 21) + bc[..., 2]
+
+
+def block_lookup(block_base, bc):
+    """index of each block coordinate (..., 3) in block_base, -1 if absent"""
+    keys = _block_keys(np.asarray(block_base, np.int64) // 4 + (1 << 20))
+    order = np.argsort(keys)
+    sk = keys[order]
+    q = _block_keys(np.asarray(bc, np.int64) + (1 << 20))
+    pos = np.clip(np.searchsorted(sk, q), 0, len(sk) - 1)
+    return np.where(sk[pos] == q, order[pos], -1)
+
+
 def force_cells(block_base, inside, force_coord):
     """(nb*64,) inside cells that share a node with a force: the 8 cells
     around each force node (spgrid_topo_opt.h:580-590)."""
-    bb = np.asarray(block_base) // 4
-    key = {tuple(b): i for i, b in enumerate(bb.tolist())}
-    out = np.zeros(len(bb) * 64, bool)
-    for f in np.asarray(force_coord).reshape(-1, 3).tolist():
-        for c in range(8):
-            x, y, z = f[0] - ((c >> 2) & 1), f[1] - ((c >> 1) & 1), f[2] - (c & 1)
-            b = key.get((x >> 2, y >> 2, z >> 2))
-            if b is not None:
-                out[b * 64 + ((x & 3) << 4) + ((y & 3) << 2) + (z & 3)] = True
+    nb = np.asarray(block_base).shape[0]
+    out = np.zeros(nb * 64, bool)
+    f = np.asarray(force_coord, np.int64).reshape(-1, 3)
+    for c in range(8):
+        cell = f - np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1])
+        b = block_lookup(block_base, cell >> 2)
+        loc = ((cell[:, 0] & 3) << 4) + ((cell[:, 1] & 3) << 2) + (cell[:, 2] & 3)
+        out[(b * 64 + loc)[b >= 0]] = True
     return out & np.asarray(inside).reshape(-1)

@@ -133,13 +133,9 @@ class CellGrid:
     """Input blocks as a cell grid with a 27-block neighbour table (torch)."""
 
     def __init__(self, prob, device="cuda"):
-        bb = prob.block_base // 4
-        key = {tuple(b): i for i, b in enumerate(bb.tolist())}
-        nbr = np.full((len(bb), 27), -1, np.int64)
-        for i, b in enumerate(bb.tolist()):
-            for s in range(27):
-                o = (s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1)
-                nbr[i, s] = key.get((b[0] + o[0], b[1] + o[1], b[2] + o[2]), -1)
+        bb = prob.block_base.astype(np.int64) // 4
+        offs = np.array([(s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1) for s in range(27)], np.int64)
+        nbr = _nb.block_lookup(prob.block_base, bb[:, None, :] + offs[None])
         self.nb = len(bb)
         self.nbr = torch.as_tensor(nbr, device=device)
         self.inside = torch.as_tensor(prob.density.reshape(-1) > 0, device=device)
