@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["shell", "ellipsoid"], default="shell",
+                    help="shell: 256^3 sphere shell (default); ellipsoid: 2048x1024x1024 ellipsoidal shell")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -68,7 +70,14 @@ def main():
     from spgrid_topo_opt_amd import workloads as W
 
     t0 = time.perf_counter()
-    prob = W.config("cfg2", n=args.n, mg_levels=args.levels)
+    if args.workload == "ellipsoid":
+        prob = W.config("cfg5")
+        workload = ("ellipsoidal shell in a 2048x1024x1024 grid, semi-axes 0.5/0.25/0.25, 16 cells thick, "
+                    "density U[0.05,1], clamp at the -x pole, point load at the +x pole")
+    else:
+        prob = W.config("cfg2", n=args.n, mg_levels=args.levels)
+        workload = (f"sphere shell {args.n}^3, cells with 0.4<=|x|<=0.5, density U[0.05,1] + 2 box filters, "
+                    "clamp at south pole, point load at north pole")
     log(f"[bench] problem {prob.name}: {prob.block_base.shape[0]} blocks, built in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
     s = _lib.Solver(prob)
@@ -109,6 +118,10 @@ def main():
     nn = info["active_nodes"]
     t_ap32 = s.bench_apply(0, args.apply_reps)
     t_ap64 = s.bench_apply(1, args.apply_reps)
+    # the same FP32 apply with FP32 row sums (the PCG operator sums in FP64)
+    os.environ["SPG_ACC64_MODES"] = "0"
+    t_ap32f = s.bench_apply(0, args.apply_reps)
+    del os.environ["SPG_ACC64_MODES"]
     t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
     t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
     log(f"[bench] V-cycle fp32 {t_vc32 * 1e3:.3f} ms, fp64 {t_vc64 * 1e3:.3f} ms")
@@ -135,8 +148,7 @@ def main():
         "dtype": "fp32-inner/fp64-outer",
         "data": "synthetic",
         "config": {
-            "workload": f"sphere shell {args.n}^3, cells with 0.4<=|x|<=0.5, density U[0.05,1] + 2 box filters, "
-                        "clamp at south pole, point load at north pole",
+            "workload": workload,
             "active_cells": info["active_cells"],
             "active_nodes": nn,
             "dofs": 3 * nn,
@@ -154,12 +166,15 @@ def main():
         "apply_mdof_s_fp32": 3 * nn / t_ap32 / 1e6,
         "apply_mdof_s_fp64": 3 * nn / t_ap64 / 1e6,
         "apply_us_fp32": t_ap32 * 1e6,
+        "apply_us_fp32_fp32_sums": t_ap32f * 1e6,
+        "apply_mdof_s_fp32_fp32_sums": 3 * nn / t_ap32f / 1e6,
         "apply_us_fp64": t_ap64 * 1e6,
         "vcycle_ms_fp32": t_vc32 * 1e3,
         "vcycle_ms_fp64": t_vc64 * 1e3,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_apply<float> (level-0 stiffness apply)",
+            "kernel": "k_apply_lds<float, double, 2> (level-0 stiffness apply of the PCG operator: FP32 in/out, "
+                      "FP64 row sums)",
             "achieved": gbs32,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -1128,7 +1128,7 @@ __global__ void __launch_bounds__(NT) k_bnd_sweeps(int n, BndOff L, ColourOrder 
 }
 
 // Dense map [G H] of a boundary phase, u_B <- G u_B + H b~ (row-major,
-// 3n x 6n, entry index node*3 + component).  Column-parallel: the phase is
+// 3n x 6n with rows padded to bmap_stride(n), entry index node*3 + component).  Column-parallel: the phase is
 // run on all unit vectors at once, lane = column (columns j < 3n start from
 // u_B = e_j with b~ = 0, columns 3n + j from u_B = 0 with b~ = e_j), so
 // every node row and its coefficients are wave-uniform.  The 16 waves of a
@@ -1141,9 +1141,9 @@ __global__ void __launch_bounds__(64 * BM_WAVES) k_bmap_build(int n, BndOff L, C
                                                             const double* __restrict__ A,
                                                             const uint8_t* __restrict__ flags,
                                                             double* __restrict__ M) {
-  const int64_t K = 6 * (int64_t)n;
+  const int64_t K = bmap_stride(n);  // row stride; columns >= 6n stay zero
   const int col = blockIdx.x * 64 + (threadIdx.x & 63);
-  const bool live = col < K;
+  const bool live = col < 6 * n;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int rows = 3 * n;
   double* Mc = M + col;
@@ -1197,8 +1197,10 @@ __global__ void __launch_bounds__(64 * BM_WAVES) k_bmap_build(int n, BndOff L, C
     }
 }
 
-// Dense boundary map apply: x = [u_B; b~] (node-major), y = [G H] x, one
-// wavefront per row with a fixed-order lane reduction.
+// Dense boundary map apply: x = [u_B; b~] (node-major), y = [G H] x.  Rows
+// are padded to a multiple of 64 entries (zeros), one wavefront per row
+// streams it with 16-byte loads, four in flight per lane, and reduces in a
+// fixed order.
 template <class T>
 __global__ void k_bmap_gather(int n, const int* __restrict__ list, const T* __restrict__ u,
                               const T* __restrict__ bt, T* __restrict__ x) {
@@ -1209,13 +1211,36 @@ __global__ void k_bmap_gather(int n, const int* __restrict__ list, const T* __re
   x[3 * n + t] = bt[i * 4 + c];
 }
 template <class T>
-__global__ void __launch_bounds__(256) k_bmap_gemv(int rows, int K, const T* __restrict__ M, const T* __restrict__ x,
+__device__ __forceinline__ T dot4(const typename Vec4<T>::type& a, const typename Vec4<T>::type& b, T acc);
+template <>
+__device__ __forceinline__ float dot4<float>(const float4& a, const float4& b, float acc) {
+  return fmaf(a.w, b.w, fmaf(a.z, b.z, fmaf(a.y, b.y, fmaf(a.x, b.x, acc))));
+}
+template <>
+__device__ __forceinline__ double dot4<double>(const double2& a, const double2& b, double acc) {
+  return fma(a.y, b.y, fma(a.x, b.x, acc));
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_bmap_gemv(int rows, int Kp, const T* __restrict__ M, const T* __restrict__ x,
                                                    T* __restrict__ y) {
+  using V = typename Vec4<T>::type;
+  constexpr int PER = 16 / sizeof(T);
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= rows) return;
-  const T* row = M + (int64_t)r * K;
-  T acc = 0;
-  for (int k = lane; k < K; k += 64) acc = fmaT(row[k], x[k], acc);
+  const V* row = reinterpret_cast<const V*>(M + (int64_t)r * Kp);
+  const V* xv = reinterpret_cast<const V*>(x);
+  const int nv = Kp / PER;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int k = lane;
+  for (; k + 192 < nv; k += 256) {
+    const V m0 = row[k], m1 = row[k + 64], m2 = row[k + 128], m3 = row[k + 192];
+    a0 = dot4<T>(m0, xv[k], a0);
+    a1 = dot4<T>(m1, xv[k + 64], a1);
+    a2 = dot4<T>(m2, xv[k + 128], a2);
+    a3 = dot4<T>(m3, xv[k + 192], a3);
+  }
+  for (; k < nv; k += 64) a0 = dot4<T>(row[k], xv[k], a0);
+  T acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
   if (lane == 0) y[r] = acc;
@@ -2109,12 +2134,13 @@ void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, co
   const T* M = BmapPtr<T>::M(L, forward);
   if (path == 0 && M && sweeps == L.bmap_sweeps) {
     const int n = L.bnd_off[8];
-    T* x = BmapPtr<T>::x(L);  // [6n] input, then [3n] output behind it
+    const int Kp = (int)bmap_stride(n);
+    T* x = BmapPtr<T>::x(L);  // [Kp] input (zero past 6n), then [3n] output behind it
     T* bt = BndPtr<T>::bt(L);
     k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, BndPtr<T>::A(L), u, b, bt);
     k_bmap_gather<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, u, bt, x);
-    k_bmap_gemv<T><<<(3 * n + 3) / 4, 256, 0, st>>>(3 * n, 6 * n, M, x, x + 6 * n);
-    k_bmap_scatter<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, x + 6 * n, u);
+    k_bmap_gemv<T><<<(3 * n + 3) / 4, 256, 0, st>>>(3 * n, Kp, M, x, x + Kp);
+    k_bmap_scatter<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, x + Kp, u);
     return;
   }
   if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, false, forward)) return;

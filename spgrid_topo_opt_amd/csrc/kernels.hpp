@@ -137,7 +137,10 @@ void launch_bnd_coef0(hipStream_t st, const LevelDev& L, const double* E, double
 void launch_bnd0_setup(hipStream_t st, const LevelDev& L, const int* cslot, const double* E, double dx, double* bE,
                        double* bD);
 void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, double* A);
-// dense FP64 map [G H] (3n x 6n, row-major) of a boundary phase on an explicit level
+// row stride of the dense boundary map of an n-node boundary set: 6n padded to 64
+__host__ __device__ inline int64_t bmap_stride(int n) { return ((int64_t)6 * n + 63) / 64 * 64; }
+// dense FP64 map [G H] (3n x 6n, row-major, stride bmap_stride(n)) of a boundary
+// phase on an explicit level; the padding columns must hold zeros
 void launch_bmap_build(hipStream_t st, const LevelDev& L, int sweeps, bool forward, double* M);
 // bottom
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,

@@ -283,11 +283,13 @@ void Solver::build_boundary_rows() {
     launch_cast_d2f(st_, R.bA64.as<double>(), R.bA32.as<float>(), (int64_t)n * 288);
     // small boundary sets of explicit levels: each phase becomes one dense map
     if (l > 0 && lay_.lv[l].is_explicit && 3 * n <= kMaxDenseBoundary && cfg_.boundary_iters > 0) {
-      const size_t entries = (size_t)3 * n * 6 * n;
+      const size_t entries = (size_t)3 * n * bmap_stride(n);
       for (int d = 0; d < 2; ++d) {
         if (!R.bmap64[d].p) {
+          // padding columns stay zero: the build writes columns < 6n only
           R.bmap64[d].alloc(entries * 8);
           R.bmap32[d].alloc(entries * 4);
+          HIPCHK(hipMemsetAsync(R.bmap64[d].p, 0, entries * 8, st_));
         }
         launch_bmap_build(st_, R.d, cfg_.boundary_iters, d == 0, R.bmap64[d].as<double>());
         launch_cast_d2f(st_, R.bmap64[d].as<double>(), R.bmap32[d].as<float>(), (int64_t)entries);
@@ -295,8 +297,11 @@ void Solver::build_boundary_rows() {
         R.d.bmap32[d] = R.bmap32[d].as<float>();
       }
       if (!R.bmx64.p) {
-        R.bmx64.alloc((size_t)9 * n * 8);
-        R.bmx32.alloc((size_t)9 * n * 4);
+        const size_t nx = (size_t)bmap_stride(n) + 3 * n;
+        R.bmx64.alloc(nx * 8);
+        R.bmx32.alloc(nx * 4);
+        HIPCHK(hipMemsetAsync(R.bmx64.p, 0, nx * 8, st_));
+        HIPCHK(hipMemsetAsync(R.bmx32.p, 0, nx * 4, st_));
       }
       R.d.bmx64 = R.bmx64.as<double>();
       R.d.bmx32 = R.bmx32.as<float>();
