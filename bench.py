@@ -71,7 +71,7 @@ def main():
 
     t0 = time.perf_counter()
     if args.workload == "ellipsoid":
-        prob = W.config("cfg5")
+        prob = W.config("cfg5", progress=lambda m: log(f"[bench] {m}"))
         workload = ("ellipsoidal shell in a 2048x1024x1024 grid, semi-axes 0.5/0.25/0.25, 16 cells thick, "
                     "density U[0.05,1], clamp at the -x pole, point load at the +x pole")
     else:

@@ -111,11 +111,14 @@ def _blocks_from_cells(cells, dens):
     bkey = cells >> 2
     ck = _pack(bkey)
     keys = [np.unique(ck)]
+    top = (cells & 3) == 3  # a shift by +1 leaves the block only from local index 3
     for a in (0, 1):
         for b in (0, 1):
             for c in (0, 1):
                 if a or b or c:
-                    keys.append(np.unique(_pack((cells + np.array([a, b, c])) >> 2)))
+                    d = np.array([a, b, c])
+                    sel = np.all(top | (d == 0), axis=1)
+                    keys.append(np.unique(_pack((cells[sel] + d) >> 2)))
     keys = np.unique(np.concatenate(keys))  # sorted: lexicographic block order (x slowest)
     bid = np.searchsorted(keys, ck)
     loc = cells & 3
@@ -264,10 +267,11 @@ def band(nx=512, ny=256, nz=256, width=24, seed=0, lo=0.05, hi=1.0, mg_levels=6,
 
 
 def ellipsoid_shell(nx=2048, ny=1024, nz=1024, semi=(0.5, 0.25, 0.25), thickness_cells=16, seed=0, lo=0.05, hi=1.0,
-                    mg_levels=6, name=None, E=E_DEFAULT, nu=NU_DEFAULT):
+                    mg_levels=6, name=None, E=E_DEFAULT, nu=NU_DEFAULT, progress=None):
     """Cells between the ellipsoid with semi-axes `semi` (domain units, centred,
     dx = 1/nx) and the one with every semi-axis reduced by thickness*dx.
-    Clamp: cells within 0.03 of the -x pole; load (0,-1,0) at the +x pole node."""
+    Clamp: cells within 0.03 of the -x pole; load (0,-1,0) at the +x pole node.
+    `progress(msg)` is called every 256 x-slices and per build stage."""
     mu, lam = lame(E, nu)
     dx = 1.0 / nx
     a, b, c = semi
@@ -285,10 +289,17 @@ def ellipsoid_shell(nx=2048, ny=1024, nz=1024, semi=(0.5, 0.25, 0.25), thickness
         if m.any():
             jk = np.argwhere(m)
             parts.append(np.concatenate([np.full((jk.shape[0], 1), i), jk], 1))
+        if progress and (i + 1) % 256 == 0:
+            progress(f"ellipsoid shell: {i + 1}/{nx} x-slices")
     ijk = np.concatenate(parts).astype(np.int64)
+    del parts
     dens = _grid_density(nx, ny, nz, seed, lo, hi, ijk)
     cc = ijk + OFFSET
+    if progress:
+        progress(f"ellipsoid shell: {ijk.shape[0]} cells, grouping into blocks")
     bb, d = _blocks_from_cells(cc, dens)
+    if progress:
+        progress(f"ellipsoid shell: {bb.shape[0]} blocks")
     cen = (ijk + 0.5) * dx - np.array([nx, ny, nz]) * dx / 2
     clamp = cc[np.linalg.norm(cen - np.array([-a, 0, 0]), axis=1) <= 0.03]
     dc, da, dv = _clamp_cells(clamp)
@@ -354,5 +365,5 @@ def config(name: str, **kw) -> Problem:
     if name == "cfg4":
         return band(512, 256, 256, 24, seed=kw.get("seed", 0), mg_levels=kw.get("mg_levels", 6))
     if name == "cfg5":
-        return ellipsoid_shell(seed=kw.get("seed", 0))
+        return ellipsoid_shell(seed=kw.get("seed", 0), progress=kw.get("progress"))
     raise KeyError(name)
