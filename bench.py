@@ -88,11 +88,30 @@ def main():
         traffic_probe(s, prob)
         return
 
+    # kernel throughput first: it does not depend on how the solve converges
+    nn = s.info()["active_nodes"]
+    t_ap32 = s.bench_apply(0, args.apply_reps)
+    t_ap64 = s.bench_apply(1, args.apply_reps)
+    # the same FP32 apply with FP32 row sums (the PCG operator sums in FP64)
+    os.environ["SPG_ACC64_MODES"] = "0"
+    t_ap32f = s.bench_apply(0, args.apply_reps)
+    del os.environ["SPG_ACC64_MODES"]
+    t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
+    t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
+    log(f"[bench] V-cycle fp32 {t_vc32 * 1e3:.3f} ms, fp64 {t_vc64 * 1e3:.3f} ms")
+    gbs32 = BYTES_PER_NODE_FP32 * nn / t_ap32 / 1e9
+    gbs64 = BYTES_PER_NODE_FP64 * nn / t_ap64 / 1e9
+    log(f"[bench] apply fp32 {t_ap32 * 1e6:.1f} us ({3 * nn / t_ap32 / 1e6:.0f} MDOF/s, {gbs32:.0f} GB/s); "
+        f"fp64 {t_ap64 * 1e6:.1f} us ({3 * nn / t_ap64 / 1e6:.0f} MDOF/s, {gbs64:.0f} GB/s)")
+
+    pr = 2 if args.workload == "ellipsoid" else 0  # per-iteration residuals on stderr for the long solve
+
     def mixed():
-        return s.solve_mixed(tolerance=args.tol, want_output=False)[1]
+        return s.solve_mixed(tolerance=args.tol, want_output=False, print_residuals=pr)[1]
 
     def fp64():
-        return s.solve(tolerance=args.tol, precision=1, max_passes=1, max_iterations=2000, want_output=False)[1]
+        return s.solve(tolerance=args.tol, precision=1, max_passes=1, max_iterations=2000, want_output=False,
+                       print_residuals=pr)[1]
 
     for _ in range(args.warmup):
         info = mixed()
@@ -114,21 +133,6 @@ def main():
     t_fp64 = time.perf_counter() - t0
     log(f"[bench] fp64 solve {t_fp64 * 1e3:.2f} ms, iterations {i64['iterations']}, "
         f"rel residual {i64['r_linf'] / i64['r0_linf']:.2e}")
-
-    nn = info["active_nodes"]
-    t_ap32 = s.bench_apply(0, args.apply_reps)
-    t_ap64 = s.bench_apply(1, args.apply_reps)
-    # the same FP32 apply with FP32 row sums (the PCG operator sums in FP64)
-    os.environ["SPG_ACC64_MODES"] = "0"
-    t_ap32f = s.bench_apply(0, args.apply_reps)
-    del os.environ["SPG_ACC64_MODES"]
-    t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
-    t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
-    log(f"[bench] V-cycle fp32 {t_vc32 * 1e3:.3f} ms, fp64 {t_vc64 * 1e3:.3f} ms")
-    gbs32 = BYTES_PER_NODE_FP32 * nn / t_ap32 / 1e9
-    gbs64 = BYTES_PER_NODE_FP64 * nn / t_ap64 / 1e9
-    log(f"[bench] apply fp32 {t_ap32 * 1e6:.1f} us ({3 * nn / t_ap32 / 1e6:.0f} MDOF/s, {gbs32:.0f} GB/s); "
-        f"fp64 {t_ap64 * 1e6:.1f} us ({3 * nn / t_ap64 / 1e6:.0f} MDOF/s, {gbs64:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:

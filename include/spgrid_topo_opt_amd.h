@@ -77,7 +77,10 @@ typedef struct {
   int32_t max_iterations;  /* PCG iterations per pass                       */
   int32_t max_passes;      /* FP64 defect-correction passes (>=1)           */
   int32_t precision;       /* 0: FP32 inner / FP64 outer (mixed), 1: FP64   */
-  int32_t reserved;
+  int32_t print_residuals; /* krylov.print_residuals: print ||r||_inf of every
+                              PCG iteration, one per line (CG/
+                              CONJUGATE_GRADIENT.cpp:34-35); 1 stdout, as the
+                              reference, 2 stderr, 0 off                     */
 } spg_solve_params;
 
 typedef struct {
@@ -139,6 +142,10 @@ int spg_sensitivity_device(spg_solver *s, const double *u_dev /*device or NULL*/
  * Vectors are exchanged in "level node order": the solver's GPU block order,
  * 64 nodes per block (local i*16+j*4+k), components xyz interleaved, i.e.
  * value[(blk*64+l)*3+c].  spg_level_nodes returns the coordinates.          */
+/* Sizes, level count, device bytes and setup time of a built solver (the
+ * solve-independent fields of spg_info; the others are zero).  No
+ * counterpart in the reference, whose solver reports these through logs.   */
+int spg_get_info(spg_solver *s, spg_info *info);
 int spg_num_levels(spg_solver *s, int32_t *levels);
 int spg_level_size(spg_solver *s, int32_t level, int64_t *nblocks);
 int spg_level_nodes(spg_solver *s, int32_t level, int32_t *xyz /*host [nb*64*3]*/, uint8_t *flags /*host [nb*64]*/);

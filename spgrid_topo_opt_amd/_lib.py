@@ -30,7 +30,7 @@ EXPORTS = [
     "fem_solve", "fem_solve_test", "spg_last_error", "spg_version", "spg_device_count", "spg_create",
     "spg_destroy", "spg_update_density", "spg_set_forces", "spg_solve", "spg_get_displacement_device",
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
-    "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel",
+    "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel", "spg_get_info",
 ]
 
 
@@ -63,7 +63,7 @@ class SolveParams(C.Structure):
         ("max_iterations", C.c_int32),
         ("max_passes", C.c_int32),
         ("precision", C.c_int32),
-        ("reserved", C.c_int32),
+        ("print_residuals", C.c_int32),
     ]
 
 
@@ -128,6 +128,7 @@ def lib():
         L.spg_level_stencils.argtypes = [vp, C.c_int32, dp]
         L.spg_bench_apply.argtypes = [vp, C.c_int32, C.c_int32, dp]
         L.spg_bench_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp]
+        L.spg_get_info.argtypes = [vp, C.POINTER(Info)]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -209,8 +210,9 @@ class Solver:
         _check(lib().spg_update_density(self.h, _dp(d)))
 
     def solve(self, tolerance=1e-6, inner_tolerance=0.0, max_iterations=500, max_passes=1, precision=0,
-              want_output=True, raise_on_error=True):
-        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, 0)
+              want_output=True, raise_on_error=True, print_residuals=0):
+        """print_residuals: 1 prints ||r||_inf of every PCG iteration to stdout, 2 to stderr"""
+        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, print_residuals)
         info = Info()
         out = np.zeros((self.nblocks, 64, 4)) if want_output else None
         rc = lib().spg_solve(self.h, C.byref(p), _dp(out), C.byref(info))
@@ -253,6 +255,12 @@ class Solver:
         n = C.c_int32(0)
         _check(lib().spg_num_levels(self.h, C.byref(n)))
         return n.value
+
+    def info(self):
+        """solve-independent sizes: levels, active nodes/cells, blocks, bottom unknowns, device bytes"""
+        info = Info()
+        _check(lib().spg_get_info(self.h, C.byref(info)))
+        return info.as_dict()
 
     def level_blocks(self, level):
         n = C.c_int64(0)

@@ -174,6 +174,15 @@ int spg_num_levels(spg_solver* s, int32_t* levels) {
   API_CATCH
 }
 
+int spg_get_info(spg_solver* s, spg_info* info) {
+  API_TRY
+  if (!s || !info) throw SpgError(SPG_E_INVALID, "null argument");
+  *info = spg_info{};
+  s->s->static_info(info);
+  return SPG_OK;
+  API_CATCH
+}
+
 int spg_level_size(spg_solver* s, int32_t level, int64_t* nblocks) {
   API_TRY
   if (!s || !nblocks) throw SpgError(SPG_E_INVALID, "null argument");
@@ -332,6 +341,7 @@ void fem_solve(void* input, void* output) {
     prm.max_passes = dci > 0 ? dci : 1;
     prm.precision = in.get_int("fp64_solve", 0) ? 1 : 0;
     prm.inner_tolerance = in.get_double("inner_tolerance", 0.0);
+    prm.print_residuals = in.krylov.print_residuals ? 1 : 0;
     spg_info info{};
     const int rc = h->s->solve(prm, &info);
     out.displacements.blocks.resize(nb);

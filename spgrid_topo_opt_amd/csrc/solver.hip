@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -526,6 +527,11 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
     sync();
     const CgScalars* h = reinterpret_cast<const CgScalars*>(hscal_);
     *last_nrm = h->nrm;
+    if (print_residuals_) {
+      FILE* o = print_residuals_ == 2 ? stderr : stdout;
+      std::fprintf(o, "%.9g\n", h->nrm);
+      std::fflush(o);
+    }
     if (h->singular) {
       rc = SPG_E_SINGULAR;
       break;
@@ -564,6 +570,7 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
   launch_linf<double>(st_, res_.as<double>(), n, part, red);
   const double r0 = read_scalar(red);
   const double target = prm.tolerance * r0;
+  print_residuals_ = prm.print_residuals;
   int total = 0, passes = 0, rc = SPG_OK;
   double rn = r0;
   bool conv = (r0 == 0.0);
@@ -600,23 +607,27 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
   launch_sumsq(st_, res_.as<double>(), n, part, red);
   const double l2 = std::sqrt(read_scalar(red));
   if (info) {
+    static_info(info);
     info->iterations = total;
     info->passes = passes;
     info->converged = conv ? 1 : 0;
-    info->levels = lay_.nlev;
     info->r0_linf = r0;
     info->r_linf = rn;
     info->r_l2 = l2;
-    info->setup_seconds = setup_s_;
     info->solve_seconds = now_s() - t0;
-    info->active_nodes = lay_.lv[0].n_active;
-    info->active_cells = lay_.n_active_cells;
-    info->gpu_blocks = lay_.lv[0].nb;
-    info->bottom_dofs = nvar_;
-    info->hbm_bytes = dev_bytes_;
   }
   if (rc == SPG_OK && !conv) rc = SPG_E_NOT_CONVERGED;
   return rc;
+}
+
+void Solver::static_info(spg_info* info) const {
+  info->levels = lay_.nlev;
+  info->setup_seconds = setup_s_;
+  info->active_nodes = lay_.lv[0].n_active;
+  info->active_cells = lay_.n_active_cells;
+  info->gpu_blocks = lay_.lv[0].nb;
+  info->bottom_dofs = nvar_;
+  info->hbm_bytes = dev_bytes_;
 }
 
 int Solver::solve(const spg_solve_params& prm, spg_info* info) {

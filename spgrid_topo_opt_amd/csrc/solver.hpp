@@ -80,6 +80,8 @@ class Solver {
   void level_stencils(int level, double* out);
   double bench_apply(int precision, int reps);
   double bench_kernel(int op, int level, int precision, int arg, int reps);
+  // sizes, levels, device bytes and setup time (the solve-independent spg_info fields)
+  void static_info(spg_info* info) const;
   const spg_config& config() const { return cfg_; }
   double setup_seconds() const { return setup_s_; }
   double device_bytes() const { return dev_bytes_; }
@@ -91,6 +93,7 @@ class Solver {
   template <class T> void level_residual(int l, const T* u, const T* b, T* r);
   template <class T> void bottom(const T* b, T* u);
   template <class T> int pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm);
+  int print_residuals_ = 0;  // 0 off, 1 stdout, 2 stderr (per PCG iteration)
   template <class T> const T* E() const;
   template <class T> T dxT() const { return (T)cfg_.dx; }
   int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool with_ud);

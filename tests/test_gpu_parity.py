@@ -353,3 +353,19 @@ def test_errors_are_reported():
     assert info["rc"] in (_lib.SPG_E_SINGULAR, _lib.SPG_E_NOT_CONVERGED)
     with pytest.raises(_lib.SpgError):
         _lib.Solver(p.with_levels(2))
+
+
+def test_print_residuals_and_static_info(box3, capfd):
+    """krylov.print_residuals prints ||r||_inf once per PCG iteration check
+    (CG/CONJUGATE_GRADIENT.cpp:34-35): iterations + 1 lines per pass;
+    spg_get_info reports the same sizes as a solve"""
+    g = box3.g
+    st = g.info()
+    _, info = g.solve(tolerance=1e-6, precision=1, max_iterations=300, want_output=False, print_residuals=2)
+    err = capfd.readouterr().err
+    lines = [float(x) for x in err.split() if x.strip()]
+    assert len(lines) == info["iterations"] + info["passes"]
+    assert lines[-1] <= lines[0]
+    for k in ("levels", "active_nodes", "active_cells", "gpu_blocks", "bottom_dofs"):
+        assert st[k] == info[k], k
+    assert st["hbm_bytes"] > 0
