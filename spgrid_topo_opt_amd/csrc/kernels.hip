@@ -1770,6 +1770,10 @@ __global__ void k_symmetrize_lower(double* A, int64_t n) {
   const int64_t i = gid % n, j = gid / n;  // A[i + j n]
   if (i < j) A[i + j * n] = A[j + i * n];
 }
+__global__ void k_set_diagonal(double* A, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[i + i * n] = v;
+}
 template <class T>
 __global__ void k_bottom_gather(int nb, const int* __restrict__ var_of, const T* __restrict__ b, T* x) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2229,6 +2233,9 @@ void launch_stencil_to_float(hipStream_t st, const double* a, float* b, int64_t 
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
                            double* A) {
   k_dense_assemble<<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, ST, var_of, n, A);
+}
+void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v) {
+  k_set_diagonal<<<nblk(n), TPB, 0, st>>>(A, n, v);
 }
 void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n) {
   k_symmetrize_lower<<<nblk(n * n), TPB, 0, st>>>(A, n);

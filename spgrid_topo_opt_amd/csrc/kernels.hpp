@@ -146,6 +146,7 @@ void launch_bmap_build(hipStream_t st, const LevelDev& L, int sweeps, bool forwa
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
                            double* A);
 void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n);
+void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v);  // A[i + i n] = v
 template <class T>
 void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, T* x);
 template <class T>

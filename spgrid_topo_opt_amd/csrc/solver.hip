@@ -238,16 +238,37 @@ void Solver::build_operators() {
   info_.alloc(16);
   HIPCHK(hipMemsetAsync(Ainv.p, 0, (size_t)nv * nv * 8, st_));
   launch_dense_assemble(st_, lv_[B].d, lv_[B].st64.as<double>(), var_of_.as<int>(), nv, Ainv.as<double>());
-  RBCHK(rocsolver_dpotrf(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
-                         info_.as<rocblas_int>()));
-  int hinfo = 0;
-  HIPCHK(hipMemcpyAsync(&hinfo, info_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
-  sync();
-  if (hinfo != 0)
-    throw SpgError(SPG_E_SINGULAR, "bottom Cholesky failed: leading minor " + std::to_string(hinfo) +
-                                       " not positive definite");
-  RBCHK(rocsolver_dpotri(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
-                         info_.as<rocblas_int>()));
+  if (nv * nv <= (int64_t)std::numeric_limits<int32_t>::max()) {
+    RBCHK(rocsolver_dpotrf(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
+                           info_.as<rocblas_int>()));
+    int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, info_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+    sync();
+    if (hinfo != 0)
+      throw SpgError(SPG_E_SINGULAR, "bottom Cholesky failed: leading minor " + std::to_string(hinfo) +
+                                         " not positive definite");
+    RBCHK(rocsolver_dpotri(rb_, rocblas_fill_lower, (rocblas_int)nv, Ainv.as<double>(), (rocblas_int)nv,
+                           info_.as<rocblas_int>()));
+  } else {
+    // more than 2^31 entries: the 32-bit interfaces overflow their offsets, so
+    // factor with the 64-bit one and form A^-1 = L^-T L^-1 = X^T X, X = L^-1
+    RBCHK(rocsolver_dpotrf_64(rb_, rocblas_fill_lower, nv, Ainv.as<double>(), nv, info_.as<int64_t>()));
+    int64_t hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, info_.p, sizeof(int64_t), hipMemcpyDeviceToHost, st_));
+    sync();
+    if (hinfo != 0)
+      throw SpgError(SPG_E_SINGULAR, "bottom Cholesky failed: leading minor " + std::to_string(hinfo) +
+                                         " not positive definite");
+    DBuf X;
+    X.alloc((size_t)nv * nv * 8);
+    HIPCHK(hipMemsetAsync(X.p, 0, (size_t)nv * nv * 8, st_));
+    launch_set_diagonal(st_, X.as<double>(), nv, 1.0);
+    RBCHK(rocblas_dtrsm_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, nv, nv, &one, Ainv.as<double>(), nv, X.as<double>(), nv));
+    RBCHK(rocblas_dsyrk_64(rb_, rocblas_fill_lower, rocblas_operation_transpose, nv, nv, &one, X.as<double>(), nv,
+                           &zero, Ainv.as<double>(), nv));
+    sync();
+  }
   launch_symmetrize_lower(st_, Ainv.as<double>(), nv);
   // packed lower-triangle tiles (FP64 for the FP64 cycle, rounded to FP32 for the FP32 cycle)
   nT_ = (int)((nv + 63) / 64);

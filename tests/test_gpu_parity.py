@@ -275,6 +275,23 @@ def test_fp64_solve_matches_oracle(levels):
     assert ig["r_linf"] <= tol * ig["r0_linf"]
 
 
+def test_bottom_with_more_than_2_31_inverse_entries():
+    """a bottom level of more than 46341 unknowns (its dense inverse has more
+    than 2^31 entries and takes the 64-bit factorization path) still solves
+    exactly: FP64 MG-PCG iterations and solution agree with the oracle"""
+    P = Pair(W.ellipsoid_shell(256, 128, 128, thickness_cells=2, mg_levels=3))
+    nv = P.g.info()["bottom_dofs"]
+    print("bottom unknowns", nv, flush=True)
+    assert nv * nv > 2 ** 31
+    u_o, r_o, io = P.o.solve(tolerance=1e-8, max_iterations=500)
+    assert io["rc"] == 0
+    out, ig = P.g.solve(tolerance=1e-8, max_iterations=500, precision=1)
+    print("oracle", io["iterations"], "gpu", ig["iterations"], flush=True)
+    assert ig["converged"] == 1
+    assert abs(ig["iterations"] - io["iterations"]) <= max(2, io["iterations"] // 20)
+    assert _solution_error(P, out, u_o) < 1e-6
+
+
 def test_mixed_solve_matches_fp64_oracle(box3):
     P = box3
     u_o, r_o, io = P.o.solve(tolerance=1e-12, max_iterations=500)
