@@ -1974,7 +1974,8 @@ static BndLists bnd_lists(const LevelDev& L) {
 static constexpr int kFusedBoundaryMax = 4096;
 // SPG_BOUNDARY_PATH selects the boundary smoother for tests: 0 (default)
 // automatic, 1 the single-workgroup global-memory kernel, 2 one launch per
-// colour step, 3 the coefficient-row LDS system also on level 0.  Paths 1
+// colour step, 3 the coefficient-row LDS system also on level 0, 4 the
+// matrix-free single-workgroup level-0 phase in either precision.  Paths 1
 // and 2 are the fallbacks for boundary sets too large for the LDS-resident
 // systems.
 static int boundary_path() {
@@ -2091,12 +2092,19 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   }
 }
+// level-0 boundary sets up to this many nodes run as one workgroup (FP32)
+constexpr int kSingleWgBoundaryMax = 4096;
 template <class T>
 void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
                          bool forward) {
   if (sweeps <= 0 || L.bnd_off[8] == 0) return;
-  const int path = boundary_path();
-  if (path == 0 && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
+  int path = boundary_path();
+  // Automatic choice, from measurement on the 256^3 shell (3655 boundary
+  // nodes, 3 sweeps): the single-workgroup phase is one CU's VALU work, the
+  // per-colour list launches spread each colour step over many CUs.  FP32:
+  // 154 vs 153 us; FP64: 235 vs 166 us.  Larger sets favour the lists more.
+  if (path == 0 && (sizeof(T) == 8 || L.bnd_off[8] > kSingleWgBoundaryMax)) path = 2;
+  if ((path == 0 || path == 4) && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
   if ((path == 0 || path == 3) && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
   const ColourOrder ord = colour_order(true, forward);
   if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {

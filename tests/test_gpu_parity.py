@@ -130,11 +130,13 @@ def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
         assert rel(got, want) < tol
 
 
-@pytest.mark.parametrize("path", ["1", "2", "3"])
+@pytest.mark.parametrize("path", ["1", "2", "3", "4"])
 def test_boundary_fallback_paths(blocky, path, monkeypatch):
-    """the other boundary smoothers (1, 2: global memory, used when the
-    boundary set does not fit in LDS; 3: coefficient rows on level 0 too):
-    same result as the oracle on every smoothed level"""
+    """every boundary smoother (1: one workgroup from global memory; 2:
+    per-colour list launches, the default for FP64 and large sets on level 0;
+    3: coefficient rows in LDS on level 0 too; 4: the matrix-free
+    one-workgroup level-0 phase in either precision): same result as the
+    oracle on every smoothed level"""
     P = blocky
     monkeypatch.setenv("SPG_BOUNDARY_PATH", path)
     rng = np.random.default_rng(11)
