@@ -214,17 +214,34 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
   const bool live = blk < nb;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
-  for (int h = lane; h < 216; h += 64) {
-    const int t = cHalo6[h];
-    const int bb = ht[w][t >> 8];
-    const int q = t & 255;
+  {
+    // 6^3 node halo (block coordinates -1..4) and 5^3 cell halo (-1..3);
+    // every pass's loads are issued before the LDS writes
+    T v[4][3], e[2];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? (A)in[vidx(bb, c, q)] : A(0);
-  }
-  for (int h = lane; h < 125; h += 64) {
-    const int t = cHalo5[h];
-    const int bb = ht[w][t >> 8];
-    he[w][h] = bb >= 0 ? (A)E[bb * 64 + (t & 255)] : A(0);
+    for (int i = 0; i < 4; ++i) {
+      const int h = lane + 64 * i;
+      const int x = h / 36 - 1, y = (h / 6) % 6 - 1, z = h % 6 - 1;
+      const int bb = h < 216 ? ht[w][ring_spoke(x, y, z)] : -1;
+      const int q = loc3(x, y, z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? in[vidx(bb, c, q)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int h = lane + 64 * i;
+      const int x = h / 25 - 1, y = (h / 5) % 5 - 1, z = h % 5 - 1;
+      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
+      e[i] = bb >= 0 ? E[bb * 64 + loc3(x, y, z)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (lane + 64 * i < 216)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = (A)v[i][c];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (lane + 64 * i < 125) he[w][lane + 64 * i] = (A)e[i];
   }
   __syncthreads();
   const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -420,12 +437,25 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_gs0_colour(int nb, int colour
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
-  for (int h = lane; h < 125; h += 64) {
-    const int t = cHaloC[colour * 125 + h];
-    const int bb = ht[w][t >> 8];
-    const int q = t & 255;
+  {
+    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
+    // coordinate p-1+h; both passes' loads are issued before one wait
+    T v[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+    for (int i = 0; i < 2; ++i) {
+      const int h = lane + 64 * i;
+      const int hx = h / 25, hy = (h / 5) % 5, hz = h % 5;
+      const int x = px - 1 + hx, y = py - 1 + hy, z = pz - 1 + hz;
+      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
+      const int q = loc3(x, y, z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (lane + 64 * i < 125)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = v[i][c];
   }
   __syncthreads();
   const int k = lane >> 3, I = lane & 7;
@@ -572,13 +602,31 @@ __global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const i
     ht[bl][sp] = blk < nb ? (sp == 13 ? blk : nbr[blk * 27 + sp]) : -1;
   }
   __syncthreads();
-  for (int h = tid; h < Q_BLOCKS * 125; h += 256) {
-    const int bl = h / 125, hh = h - bl * 125;
-    const int t = cHaloC[colour * 125 + hh];
-    const int bb = ht[bl][t >> 8];
-    const int q = t & 255;
+  {
+    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
+    // coordinate p-1+h; all four passes' loads are issued before one wait
+    constexpr int NP = (Q_BLOCKS * 125 + 255) / 256;
+    T v[NP][3];
+    int dst[NP];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hu[bl][c][hh] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+    for (int i = 0; i < NP; ++i) {
+      const int h = tid + 256 * i;
+      const bool ok = h < Q_BLOCKS * 125;
+      const int bl = ok ? h / 125 : 0, hh = h - bl * 125;
+      const int hx = hh / 25, hy = (hh / 5) % 5, hz = hh % 5;
+      const int x = px - 1 + hx, y = py - 1 + hy, z = pz - 1 + hz;
+      const int bb = ok ? ht[bl][ring_spoke(x, y, z)] : -1;
+      const int q = loc3(x, y, z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+      dst[i] = ok ? bl * 384 + hh : -1;
+    }
+    T* hf = &hu[0][0][0];
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+      if (dst[i] >= 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hf[dst[i] + 128 * c] = v[i][c];
   }
   __syncthreads();
   const int bl = tid >> 5, k = (tid >> 2) & 7, q = tid & 3;
@@ -753,12 +801,24 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   __syncthreads();
-  for (int h = lane; h < 125; h += 64) {
-    const int t = cHaloC[colour * 125 + h];
-    const int bb = ht[w][t >> 8];
-    const int q = t & 255;
+  {
+    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
+    // coordinate p-1+h; both passes' loads are issued before one wait
+    T v[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hu[w][c][h] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+    for (int i = 0; i < 2; ++i) {
+      const int h = lane + 64 * i;
+      const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
+      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
+      const int q = loc3(x, y, z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (lane + 64 * i < 125)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = v[i][c];
   }
   __syncthreads();
   const int k = lane >> 3, g = lane & 7;
@@ -2081,8 +2141,9 @@ void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, dou
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
   const char* e = std::getenv("SPG_GS_KERNEL");
-  // measured on gfx950: the quad kernel is fastest in FP32, the octet kernel in FP64
-  const int kind = e && std::atoi(e) > 0 ? std::atoi(e) : (sizeof(T) == 4 ? 3 : 1);
+  // measured on gfx950 (256^3 shell, per 8-colour sweep): quad 374 / 535 us,
+  // octet 455 / 580 us, corner-per-wave 533 / 898 us (FP32 / FP64)
+  const int kind = e && std::atoi(e) > 0 ? std::atoi(e) : 3;
   if (kind == 2) {
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   } else if (kind == 3) {
