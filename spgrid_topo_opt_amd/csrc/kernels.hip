@@ -206,8 +206,9 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
                                                              const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ E, T dx, const T* __restrict__ in,
                                                              const T* __restrict__ b, T* __restrict__ out) {
-  __shared__ A hu[AP_WAVES][3][216];
-  __shared__ A he[AP_WAVES][128];
+  // the halo stays in the input type; values widen to A when read
+  __shared__ T hu[AP_WAVES][3][216];
+  __shared__ T he[AP_WAVES][128];
   __shared__ int ht[AP_WAVES][28];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * AP_WAVES + w;
@@ -238,10 +239,10 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
     for (int i = 0; i < 4; ++i)
       if (lane + 64 * i < 216)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = (A)v[i][c];
+        for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = v[i][c];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
-      if (lane + 64 * i < 125) he[w][lane + 64 * i] = (A)e[i];
+      if (lane + 64 * i < 125) he[w][lane + 64 * i] = e[i];
   }
   __syncthreads();
   const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
@@ -250,13 +251,13 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
   for (int s = 0; s < 27; ++s) {
     const int h = (li + s / 9) * 36 + (lj + (s / 3) % 3) * 6 + (lk + s % 3);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) uu[s][c] = hu[w][c][h];
+    for (int c = 0; c < 3; ++c) uu[s][c] = (A)hu[w][c][h];
   }
   A f0 = 0, f1 = 0, f2 = 0;
 #pragma unroll
   for (int I = 0; I < 8; ++I) {
     const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
-    const A e = he[w][(li - ix + 1) * 25 + (lj - iy + 1) * 5 + (lk - iz + 1)];
+    const A e = (A)he[w][(li - ix + 1) * 25 + (lj - iy + 1) * 5 + (lk - iz + 1)];
     A t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
     for (int J = 0; J < 8; ++J) {
