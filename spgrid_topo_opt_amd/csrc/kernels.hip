@@ -739,13 +739,47 @@ __global__ void __launch_bounds__(BND_THREADS) k_gs0_boundary(BndLists L, Colour
     }
 }
 // one colour of a boundary list over many workgroups (long lists)
+// One level-0 colour step over a list of nodes (the boundary lists), one
+// octet per node like gs0_octet_global, with the 27 neighbour slots of each
+// list node read from a precomputed table (k_bnd_nslot) instead of walking
+// the block-neighbour table.
 template <class T>
-__global__ void __launch_bounds__(256) k_gs0_list(const int* __restrict__ list, int n, const int* __restrict__ nbr,
-                                                  const uint8_t* __restrict__ flags, const T* __restrict__ E, T dx,
-                                                  T* __restrict__ u, const T* __restrict__ b) {
-  const int i = blockIdx.x * 32 + (threadIdx.x >> 3);
+__global__ void __launch_bounds__(256) k_gs0_list_tab(const int* __restrict__ list, const int* __restrict__ nslot,
+                                                      int n, const uint8_t* __restrict__ flags,
+                                                      const T* __restrict__ E, T dx, T* __restrict__ u,
+                                                      const T* __restrict__ b) {
+  const int i = blockIdx.x * 32 + (threadIdx.x >> 3), I = threadIdx.x & 7;
   const bool live = i < n;
-  gs0_octet_global<T>(live, live ? list[i] : 0, threadIdx.x & 7, nbr, flags, E, dx, u, b);
+  const int* ns = nslot + (int64_t)(live ? i : 0) * 27;
+  const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
+  const int cs = live ? ns[(1 - ix) * 9 + (1 - iy) * 3 + (1 - iz)] : -1;  // cell with min corner node - I
+  const T e = cs >= 0 ? E[cs] : T(0);
+  auto U = [&](int J, T& v0, T& v1, T& v2) {
+    const int sp = (((J >> 2) & 1) - ix + 1) * 9 + (((J >> 1) & 1) - iy + 1) * 3 + ((J & 1) - iz + 1);
+    const int q = live ? ns[sp] : -1;
+    v0 = q >= 0 ? u[vidx(q >> 6, 0, q & 63)] : T(0);
+    v1 = q >= 0 ? u[vidx(q >> 6, 1, q & 63)] : T(0);
+    v2 = q >= 0 ? u[vidx(q >> 6, 2, q & 63)] : T(0);
+  };
+  T f0, f1, f2, d[6];
+  corner_row<T>(I, e, U, f0, f1, f2);
+  octet_row_and_diag<T>(I, e, dx, f0, f1, f2, d);
+  if (I == 0 && live) {
+    const int node = list[i], blk = node >> 6, l = node & 63;
+    gs_node_update<T>(flags[node], d, f0, f1, f2, blk, l, u[vidx(blk, 0, l)], u[vidx(blk, 1, l)],
+                      u[vidx(blk, 2, l)], b, u);
+  }
+}
+// slot (block*64 + local) of the 27 neighbours of every list node, -1 if absent
+__global__ void k_bnd_nslot(int n, const int* __restrict__ list, const int* __restrict__ nbr, int* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 27) return;
+  const int i = t / 27, sp0 = t - 27 * i;
+  const int node = list[i], blk = node >> 6, l = node & 63;
+  const int x = (l >> 4) + sp0 / 9 - 1, y = ((l >> 2) & 3) + (sp0 / 3) % 3 - 1, z = (l & 3) + sp0 % 3 - 1;
+  const int sp = ring_spoke(x, y, z);
+  const int bb = sp == 13 ? blk : nbr[blk * 27 + sp];
+  out[t] = bb >= 0 ? bb * 64 + loc3(x, y, z) : -1;
 }
 
 // --------------------------------------------------------- explicit levels
@@ -2034,11 +2068,11 @@ static BndLists bnd_lists(const LevelDev& L) {
 // a boundary phase runs in one workgroup while no colour list is longer than this
 static constexpr int kFusedBoundaryMax = 4096;
 // SPG_BOUNDARY_PATH selects the boundary smoother for tests: 0 (default)
-// automatic, 1 the single-workgroup global-memory kernel, 2 one launch per
-// colour step, 3 the coefficient-row LDS system also on level 0, 4 the
-// matrix-free single-workgroup level-0 phase in either precision.  Paths 1
-// and 2 are the fallbacks for boundary sets too large for the LDS-resident
-// systems.
+// automatic (level 0: one launch per colour step; explicit levels: the
+// dense map, else the LDS coefficient-row system), 1 the single-workgroup
+// global-memory kernel, 2 one launch per colour step, 3 the coefficient-row
+// LDS system also on level 0, 4 the matrix-free single-workgroup level-0
+// phase.
 static int boundary_path() {
   const char* e = std::getenv("SPG_BOUNDARY_PATH");
   return e ? std::atoi(e) : 0;
@@ -2154,8 +2188,6 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   }
 }
-// level-0 boundary sets up to this many nodes run as one workgroup (FP32)
-constexpr int kSingleWgBoundaryMax = 4096;
 template <class T>
 void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
                          bool forward) {
@@ -2164,8 +2196,8 @@ void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T*
   // Automatic choice, from measurement on the 256^3 shell (3655 boundary
   // nodes, 3 sweeps): the single-workgroup phase is one CU's VALU work, the
   // per-colour list launches spread each colour step over many CUs.  FP32:
-  // 154 vs 153 us; FP64: 235 vs 166 us.  Larger sets favour the lists more.
-  if (path == 0 && (sizeof(T) == 8 || L.bnd_off[8] > kSingleWgBoundaryMax)) path = 2;
+  // 154 vs 144 us; FP64: 235 vs 162 us.  Larger sets favour the lists more.
+  if (path == 0) path = 2;
   if ((path == 0 || path == 4) && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
   if ((path == 0 || path == 3) && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
   const ColourOrder ord = colour_order(true, forward);
@@ -2176,7 +2208,9 @@ void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T*
   for (int sw = 0; sw < sweeps; ++sw)
     for (int i = 0; i < 8; ++i) {
       const int c = ord.at(i), n = L.bnd_off[c + 1] - L.bnd_off[c];
-      if (n > 0) k_gs0_list<T><<<(n + 31) / 32, 256, 0, st>>>(L.bnd + L.bnd_off[c], n, L.nbr, L.flags, E, dx, u, b);
+      if (n > 0)
+        k_gs0_list_tab<T><<<(n + 31) / 32, 256, 0, st>>>(L.bnd + L.bnd_off[c], L.bnslot + (int64_t)L.bnd_off[c] * 27,
+                                                         n, L.flags, E, dx, u, b);
     }
 }
 template <class T>
@@ -2303,6 +2337,10 @@ void launch_stencil_to_float(hipStream_t st, const double* a, float* b, int64_t 
 void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, const int* var_of, int64_t n,
                            double* A) {
   k_dense_assemble<<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, ST, var_of, n, A);
+}
+void launch_bnd_nslot(hipStream_t st, const LevelDev& L, int* out) {
+  const int n = L.bnd_off[8];
+  if (n > 0) k_bnd_nslot<<<nblk((int64_t)n * 27), TPB, 0, st>>>(n, L.bnd, L.nbr, out);
 }
 void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v) {
   k_set_diagonal<<<nblk(n), TPB, 0, st>>>(A, n, v);

@@ -45,6 +45,8 @@ struct LevelDev {
   float* bmx32 = nullptr;
   double* bmx64 = nullptr;
   int bmap_sweeps = 0;
+  // level 0: slots of the 27 neighbours of every boundary node ([n*27], -1 absent)
+  const int* bnslot = nullptr;
 };
 
 // Colour of the i-th step of a Gauss-Seidel sweep: level 0 runs the colours
@@ -147,6 +149,7 @@ void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, 
                            double* A);
 void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n);
 void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v);  // A[i + i n] = v
+void launch_bnd_nslot(hipStream_t st, const LevelDev& L, int* out);  // fills LevelDev::bnslot
 template <class T>
 void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, T* x);
 template <class T>

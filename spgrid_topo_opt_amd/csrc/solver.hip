@@ -155,6 +155,11 @@ void Solver::upload_layout() {
     R.d.bt64 = R.bt64.as<double>();
     R.d.bt32 = R.bt32.as<float>();
     R.d.nbr = R.nbr.as<int>();
+    if (l == 0 && !bnd.empty()) {
+      R.bnslot.alloc(bnd.size() * 27 * 4);
+      launch_bnd_nslot(st_, R.d, R.bnslot.as<int>());
+      R.d.bnslot = R.bnslot.as<int>();
+    }
     R.d.flags = R.flags.as<uint8_t>();
     R.d.base = R.base.as<int>();
     R.d.rsrc = R.rsrc.as<int>();

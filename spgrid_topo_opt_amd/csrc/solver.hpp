@@ -51,6 +51,7 @@ struct LevelRT {
   DBuf bci, bslot, bA64, bA32, bt64, bt32;  // boundary system
   DBuf b0cslot, b0node, b0E64, b0E32, b0D64, b0D32;  // level-0 matrix-free boundary system
   DBuf bmap64[2], bmap32[2], bmx64, bmx32;           // dense boundary maps (explicit levels)
+  DBuf bnslot;                                       // level 0: neighbour slots of boundary nodes
   DBuf u32, b32, r32, u64, b64, r64;
   int64_t n3 = 0;  // vector length nb*192
   template <class T> T* u() const;
