@@ -210,30 +210,33 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
   __shared__ T hu[AP_WAVES][3][216];
   __shared__ T he[AP_WAVES][128];
   __shared__ int ht[AP_WAVES][28];
+  __shared__ int htab6[216], htab5[128];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * AP_WAVES + w;
   const bool live = blk < nb;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  if (threadIdx.x < 216) htab6[threadIdx.x] = cHalo6[threadIdx.x];
+  if (threadIdx.x < 125) htab5[threadIdx.x] = cHalo5[threadIdx.x];
   __syncthreads();
   {
-    // 6^3 node halo (block coordinates -1..4) and 5^3 cell halo (-1..3);
-    // every pass's loads are issued before the LDS writes
+    // 6^3 node halo (block coordinates -1..4) and 5^3 cell halo (-1..3)
+    // from the tables in LDS; every pass's loads are issued before the LDS
+    // writes
     T v[4][3], e[2];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int h = lane + 64 * i;
-      const int x = h / 36 - 1, y = (h / 6) % 6 - 1, z = h % 6 - 1;
-      const int bb = h < 216 ? ht[w][ring_spoke(x, y, z)] : -1;
-      const int q = loc3(x, y, z);
+      const int t = h < 216 ? htab6[h] : 0;
+      const int bb = h < 216 ? ht[w][t >> 8] : -1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? in[vidx(bb, c, q)] : T(0);
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? in[vidx(bb, c, t & 255)] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int h = lane + 64 * i;
-      const int x = h / 25 - 1, y = (h / 5) % 5 - 1, z = h % 5 - 1;
-      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
-      e[i] = bb >= 0 ? E[bb * 64 + loc3(x, y, z)] : T(0);
+      const int t = h < 125 ? htab5[h] : 0;
+      const int bb = h < 125 ? ht[w][t >> 8] : -1;
+      e[i] = bb >= 0 ? E[bb * 64 + (t & 255)] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -432,25 +435,25 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_gs0_colour(int nb, int colour
                                                               const T* __restrict__ b) {
   __shared__ T hu[GS_WAVES][3][128];
   __shared__ int ht[GS_WAVES][28];
+  __shared__ int htab[128];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * GS_WAVES + w;
   const bool live = blk < nb;
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  if (threadIdx.x < 125) htab[threadIdx.x] = cHaloC[colour * 125 + threadIdx.x];
   __syncthreads();
   {
-    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
-    // coordinate p-1+h; both passes' loads are issued before one wait
+    // the 5^3 halo of colour (px,py,pz) from the table in LDS; both passes'
+    // loads are issued before one wait
     T v[2][3];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int h = lane + 64 * i;
-      const int hx = h / 25, hy = (h / 5) % 5, hz = h % 5;
-      const int x = px - 1 + hx, y = py - 1 + hy, z = pz - 1 + hz;
-      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
-      const int q = loc3(x, y, z);
+      const int t = h < 125 ? htab[h] : 0;
+      const int bb = h < 125 ? ht[w][t >> 8] : -1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, t & 255)] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -595,6 +598,7 @@ __global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const i
                                                      T dx, T* __restrict__ u, const T* __restrict__ b) {
   __shared__ T hu[Q_BLOCKS][3][128];
   __shared__ int ht[Q_BLOCKS][27];
+  __shared__ int htab[128];
   const int tid = threadIdx.x;
   const int blk0 = xcd_chunk(blockIdx.x, gridDim.x) * Q_BLOCKS;
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
@@ -602,32 +606,26 @@ __global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const i
     const int bl = tid / 27, sp = tid - bl * 27, blk = blk0 + bl;
     ht[bl][sp] = blk < nb ? (sp == 13 ? blk : nbr[blk * 27 + sp]) : -1;
   }
+  if (tid < 125) htab[tid] = cHaloC[colour * 125 + tid];
   __syncthreads();
   {
-    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
-    // coordinate p-1+h; all four passes' loads are issued before one wait
-    constexpr int NP = (Q_BLOCKS * 125 + 255) / 256;
-    T v[NP][3];
-    int dst[NP];
+    // 32 threads per block stage its 5^3 halo (4 passes, table in LDS); all
+    // loads are issued before one wait
+    const int sb = tid >> 5, r = tid & 31;
+    T v[4][3];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const int h = tid + 256 * i;
-      const bool ok = h < Q_BLOCKS * 125;
-      const int bl = ok ? h / 125 : 0, hh = h - bl * 125;
-      const int hx = hh / 25, hy = (hh / 5) % 5, hz = hh % 5;
-      const int x = px - 1 + hx, y = py - 1 + hy, z = pz - 1 + hz;
-      const int bb = ok ? ht[bl][ring_spoke(x, y, z)] : -1;
-      const int q = loc3(x, y, z);
+    for (int i = 0; i < 4; ++i) {
+      const int hh = r + 32 * i;
+      const int t = hh < 125 ? htab[hh] : 0;
+      const int bb = hh < 125 ? ht[sb][t >> 8] : -1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
-      dst[i] = ok ? bl * 384 + hh : -1;
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, t & 255)] : T(0);
     }
-    T* hf = &hu[0][0][0];
 #pragma unroll
-    for (int i = 0; i < NP; ++i)
-      if (dst[i] >= 0)
+    for (int i = 0; i < 4; ++i)
+      if (r + 32 * i < 125)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) hf[dst[i] + 128 * c] = v[i][c];
+        for (int c = 0; c < 3; ++c) hu[sb][c][r + 32 * i] = v[i][c];
   }
   __syncthreads();
   const int bl = tid >> 5, k = (tid >> 2) & 7, q = tid & 3;
@@ -828,6 +826,7 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
                                                              const T* __restrict__ b, T* __restrict__ r) {
   __shared__ T hu[GS_WAVES][3][128];
   __shared__ int ht[GS_WAVES][28];
+  __shared__ int htab[GS_WAVES][128];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wid = xcd_chunk(blockIdx.x, gridDim.x) * GS_WAVES + w;
   const int blk = MODE == 0 ? wid : (wid >> 3);
@@ -835,19 +834,19 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
   const bool live = blk < nb;
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  for (int h = lane; h < 125; h += 64) htab[w][h] = cHaloC[colour * 125 + h];
   __syncthreads();
   {
-    // the 5^3 halo of colour (px,py,pz): halo coordinate h <-> block
-    // coordinate p-1+h; both passes' loads are issued before one wait
+    // the 5^3 halo of colour (px,py,pz) from the table in LDS; both passes'
+    // loads are issued before one wait
     T v[2][3];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int h = lane + 64 * i;
-      const int x = px - 1 + h / 25, y = py - 1 + (h / 5) % 5, z = pz - 1 + h % 5;
-      const int bb = h < 125 ? ht[w][ring_spoke(x, y, z)] : -1;
-      const int q = loc3(x, y, z);
+      const int t = h < 125 ? htab[w][h] : 0;
+      const int bb = h < 125 ? ht[w][t >> 8] : -1;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, q)] : T(0);
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, t & 255)] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
