@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--apply-reps", type=int, default=50)
+    ap.add_argument("--max-iterations", type=int, default=500, help="PCG iterations per pass (FP64 solve: 4x)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the FP32 apply and a 512 MiB copy (5 launches each) for rocprofv3 --pmc passes")
     ap.add_argument("--opt-loop", action="store_true", help="time the OC optimization loop on the 512x256x256 band")
@@ -107,11 +108,12 @@ def main():
     pr = 2 if args.workload == "ellipsoid" else 0  # per-iteration residuals on stderr for the long solve
 
     def mixed():
-        return s.solve_mixed(tolerance=args.tol, want_output=False, print_residuals=pr)[1]
+        return s.solve_mixed(tolerance=args.tol, want_output=False, print_residuals=pr,
+                             max_iterations=args.max_iterations, raise_on_error=False)[1]
 
     def fp64():
-        return s.solve(tolerance=args.tol, precision=1, max_passes=1, max_iterations=2000, want_output=False,
-                       print_residuals=pr)[1]
+        return s.solve(tolerance=args.tol, precision=1, max_passes=1, max_iterations=4 * args.max_iterations,
+                       want_output=False, print_residuals=pr, raise_on_error=False)[1]
 
     for _ in range(args.warmup):
         info = mixed()
@@ -133,6 +135,12 @@ def main():
     t_fp64 = time.perf_counter() - t0
     log(f"[bench] fp64 solve {t_fp64 * 1e3:.2f} ms, iterations {i64['iterations']}, "
         f"rel residual {i64['r_linf'] / i64['r0_linf']:.2e}")
+
+    # accuracy against a tight FP64 reference solve (10^-10) on the same
+    # problem: relative L2 displacement error and relative compliance error
+    acc = accuracy_report(s, args, pr)
+    log(f"[bench] vs FP64 reference: mixed u err {acc['mixed']['u_rel_l2']:.2e}, c err {acc['mixed']['c_rel']:.2e}; "
+        f"fp64 u err {acc['fp64']['u_rel_l2']:.2e}, c err {acc['fp64']['c_rel']:.2e}")
 
     cpu = None
     if not args.no_cpu:
@@ -165,6 +173,9 @@ def main():
         "passes_mixed": info["passes"],
         "iterations_fp64": i64["iterations"],
         "rel_residual_mixed": info["r_linf"] / info["r0_linf"],
+        "converged_mixed": bool(info["converged"]),
+        "converged_fp64": bool(i64["converged"]),
+        "rel_residual_fp64": i64["r_linf"] / i64["r0_linf"],
         "setup_s": setup_s,
         "hbm_footprint_gb": info["hbm_bytes"] / 1e9,
         "apply_mdof_s_fp32": 3 * nn / t_ap32 / 1e6,
@@ -186,8 +197,38 @@ def main():
             "traffic": measured_traffic("k_apply_lds<float, double, 2>", nn),
         },
         "cpu_baseline": cpu,
+        "accuracy_vs_fp64_reference": acc,
     }
     print(json.dumps(out), flush=True)
+
+
+def accuracy_report(s, args, pr):
+    """Relative L2 displacement error and relative compliance error of the
+    mixed and the FP64 solve (both to args.tol) against an FP64 solve to
+    10^-10; the compliance is the sensitivity kernel's objective
+    (sum_e u_e^T K0 u_e E(rho_e))."""
+    import torch
+
+    n = s.nblocks * 64 * 3
+    dc = torch.empty(s.nblocks * 64, dtype=torch.float64, device="cuda")
+
+    def run(**kw):
+        _, info = s.solve(want_output=False, raise_on_error=False, print_residuals=pr, **kw)
+        u = torch.empty(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        s.displacement_device(u.data_ptr())
+        c = s.sensitivity_device(dc.data_ptr(), u_ptr=u.data_ptr())
+        return u, c, info
+
+    u_ref, c_ref, i_ref = run(tolerance=1e-10, precision=1, max_passes=1, max_iterations=8 * args.max_iterations)
+    out = {"reference": {"tolerance": 1e-10, "iterations": i_ref["iterations"], "converged": bool(i_ref["converged"]),
+                         "compliance": c_ref}}
+    for name, kw in (("mixed", dict(precision=0, max_passes=30, max_iterations=args.max_iterations)),
+                     ("fp64", dict(precision=1, max_passes=1, max_iterations=4 * args.max_iterations))):
+        u, c, info = run(tolerance=args.tol, **kw)
+        out[name] = {"u_rel_l2": float(torch.linalg.vector_norm(u - u_ref) / torch.linalg.vector_norm(u_ref)),
+                     "c_rel": abs(c - c_ref) / abs(c_ref), "iterations": info["iterations"]}
+    return out
 
 
 COPY_BYTES = 512 << 20
