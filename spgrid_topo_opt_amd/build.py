@@ -31,7 +31,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     cc = hipcc()
     common = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result",
-              "-Wno-unused-function"]
+              "-Wno-unused-function", "-pthread"]
     procs = []
     objs = []
     for s in SOURCES:
@@ -57,7 +57,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         raise RuntimeError("HIP build failed:\n" + msg)
     tmp = LIB + ".tmp"
     link = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp] + objs + [
-        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+        "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-pthread", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(link), flush=True)
     r = subprocess.run(link, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

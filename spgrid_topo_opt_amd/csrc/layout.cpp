@@ -2,8 +2,11 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <cmath>
+#include <exception>
+#include <thread>
 #include <unordered_set>
 
 namespace spg {
@@ -89,11 +92,9 @@ std::vector<double> galerkin_position_matrices(int depth, const double* K0) {
 
 // ------------------------------------------------------------- host levels
 int32_t HostLevel::get(int bx, int by, int bz) {
-  uint64_t k = bkey(bx, by, bz);
-  auto it = map.find(k);
-  if (it != map.end()) return it->second;
-  int32_t id = (int32_t)alloc.size();
-  map.emplace(k, id);
+  const int32_t id = (int32_t)alloc.size();
+  const int32_t got = map.insert(bx, by, bz, id);
+  if (got != id) return got;
   bc.push_back(bx);
   bc.push_back(by);
   bc.push_back(bz);
@@ -114,6 +115,34 @@ void HostLevel::mark_page(int x, int y, int z) {
 
 static inline int loc(int x, int y, int z) { return ((x & 3) << 4) | ((y & 3) << 2) | (z & 3); }
 
+// f(begin, end) over [0, n) on up to 16 threads (blocks are independent in
+// every loop that uses it); the first exception is rethrown after the join
+template <class F>
+static void parallel_ranges(int64_t n, F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::min(hw, 16u)), (n + 4095) / 4096);
+  if (nt <= 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nt);
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t a = t * chunk, b = std::min(n, a + chunk);
+    th.emplace_back([&, t, a, b] {
+      try {
+        if (a < b) f(a, b);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 // Flag_Nodes (Flaging.h:127-142): cells of allocated blocks push their
 // ACTIVE / DIRICHLET bits (>>4) to their 8 nodes.
 static void flag_nodes(HostLevel& H) {
@@ -121,14 +150,16 @@ static void flag_nodes(HostLevel& H) {
   for (size_t b = 0; b < nb0; ++b) {
     if (!H.alloc[b]) continue;
     int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
+    int32_t nb8[8] = {-2, -2, -2, -2, -2, -2, -2, -2};  // this block and its + neighbours, looked up once
     for (int l = 0; l < 64; ++l) {
       uint16_t nf = (uint16_t)((H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL)) >> 4);
       if (!nf) continue;
       int x = bx * 4 + (l >> 4), y = by * 4 + ((l >> 2) & 3), z = bz * 4 + (l & 3);
       for (int c = 0; c < 8; ++c) {
         int X = x + ((c >> 2) & 1), Y = y + ((c >> 1) & 1), Z = z + (c & 1);
-        int32_t t = H.get(X >> 2, Y >> 2, Z >> 2);
-        H.nflag[(size_t)t * 64 + loc(X, Y, Z)] |= nf;
+        const int o = (((X >> 2) - bx) << 2) | (((Y >> 2) - by) << 1) | ((Z >> 2) - bz);
+        if (nb8[o] == -2) nb8[o] = H.get(X >> 2, Y >> 2, Z >> 2);
+        H.nflag[(size_t)nb8[o] * 64 + loc(X, Y, Z)] |= nf;
       }
     }
   }
@@ -199,7 +230,29 @@ Layout build_layout(const ProblemDesc& p) {
 
   // ---- level 0: pages of the input blocks (SPGrid_CPU_Solver.cpp:107-138)
   HostLevel& H0 = H[0];
-  H0.map.reserve((size_t)p.nblocks * 3);
+  // block boxes: level 0 from the input blocks (pages add at most one block
+  // in z), each coarse level from the map (x + B) >> 1 of the finer box; a
+  // margin of 2 blocks, anything outside falls back to hashing
+  std::vector<std::array<int, 6>> box(out.nlev);
+  {
+    std::array<int, 6> b = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int64_t i = 0; i < p.nblocks; ++i)
+      for (int d = 0; d < 3; ++d) {
+        b[d] = std::min(b[d], p.block_base[3 * i + d] >> 2);
+        b[3 + d] = std::max(b[3 + d], p.block_base[3 * i + d] >> 2);
+      }
+    for (int d = 0; d < 3; ++d) {
+      b[d] -= 2;
+      b[3 + d] += 2;
+    }
+    box[0] = b;
+    for (int l = 1; l < out.nlev; ++l)
+      for (int d = 0; d < 3; ++d) {
+        box[l][d] = (((box[l - 1][d] * 4 + kB[d]) >> 1) >> 2) - 2;
+        box[l][3 + d] = (((box[l - 1][3 + d] * 4 + 3 + kB[d]) >> 1) >> 2) + 2;
+      }
+    for (int l = 0; l < out.nlev; ++l) H[l].map.set_box(box[l].data(), box[l].data() + 3);
+  }
   std::unordered_set<uint64_t> pages;
   for (int64_t b = 0; b < p.nblocks; ++b) {
     const int32_t* bb = p.block_base + 3 * b;
@@ -221,34 +274,49 @@ Layout build_layout(const ProblemDesc& p) {
   }
   // cell moduli (SIMP) and ACTIVE_CELL (Flaging.h:93-106)
   std::vector<double> E0(H0.alloc.size() * 64, 0.0), R0(H0.alloc.size() * 64, 0.0);
-  for (int64_t b = 0; b < p.nblocks; ++b) {
-    const int32_t* bb = p.block_base + 3 * b;
-    for (int l = 0; l < 64; ++l) {
-      int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
-      int32_t hb = H0.find(x >> 2, y >> 2, z >> 2);
-      if (hb < 0 || !H0.alloc[hb]) continue;  // cells outside set pages are dropped
-      double d = p.density[b * 64 + l];
-      if (std::isnan(d)) throw LayoutError("density contains NaN", -6);
-      size_t s = (size_t)hb * 64 + loc(x, y, z);
-      R0[s] = d;
-      if (d > 0) {
-        double e = stiffness_mapping(p.penalty, p.min_stiffness, p.min_density, d);
-        E0[s] = e;
-        if (e * p.mu != 0.0) H0.cflag[s] |= F_ACTIVE_CELL;
+  parallel_ranges(p.nblocks, [&](int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; ++b) {
+      const int32_t* bb = p.block_base + 3 * b;
+      // a 4-aligned input block (the caller's layout) lies in one level-0 block
+      const bool aligned = ((bb[0] | bb[1] | bb[2]) & 3) == 0;
+      const int32_t hb0 = aligned ? H0.find(bb[0] >> 2, bb[1] >> 2, bb[2] >> 2) : -1;
+      if (aligned && (hb0 < 0 || !H0.alloc[hb0])) continue;
+      for (int l = 0; l < 64; ++l) {
+        int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
+        const int32_t hb = aligned ? hb0 : H0.find(x >> 2, y >> 2, z >> 2);
+        if (hb < 0 || !H0.alloc[hb]) continue;  // cells outside set pages are dropped
+        double d = p.density[b * 64 + l];
+        if (std::isnan(d)) throw LayoutError("density contains NaN", -6);
+        size_t s = (size_t)hb * 64 + loc(x, y, z);
+        R0[s] = d;
+        if (d > 0) {
+          double e = stiffness_mapping(p.penalty, p.min_stiffness, p.min_density, d);
+          E0[s] = e;
+          if (e * p.mu != 0.0) H0.cflag[s] |= F_ACTIVE_CELL;
+        }
       }
     }
-  }
-  // block activation check (Validate_Blocks.h:46-69)
-  for (size_t b = 0; b < H0.alloc.size(); ++b)
+  });
+  // block activation check (Validate_Blocks.h:46-69); a cell's nodes lie in
+  // its own block or its + neighbours, whose allocation is looked up once
+  parallel_ranges((int64_t)H0.alloc.size(), [&](int64_t b0, int64_t b1) {
+  for (int64_t b = b0; b < b1; ++b) {
+    int8_t al8[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     for (int l = 0; l < 64; ++l) {
       if (!(H0.cflag[b * 64 + l] & F_ACTIVE_CELL)) continue;
       int x = H0.bc[3 * b] * 4 + (l >> 4), y = H0.bc[3 * b + 1] * 4 + ((l >> 2) & 3), z = H0.bc[3 * b + 2] * 4 + (l & 3);
-      for (int c = 1; c < 8; ++c)
-        if (!H0.node_allocated(x + ((c >> 2) & 1), y + ((c >> 1) & 1), z + (c & 1)))
+      for (int c = 1; c < 8; ++c) {
+        const int X = x + ((c >> 2) & 1), Y = y + ((c >> 1) & 1), Z = z + (c & 1);
+        const int o = (((X >> 2) - H0.bc[3 * b]) << 2) | (((Y >> 2) - H0.bc[3 * b + 1]) << 1) | ((Z >> 2) - H0.bc[3 * b + 2]);
+        if (al8[o] < 0) al8[o] = H0.node_allocated(X, Y, Z) ? 1 : 0;
+        if (!al8[o])
           throw LayoutError("block activation check failed at cell (" + std::to_string(x) + "," + std::to_string(y) +
                                 "," + std::to_string(z) + "): its nodes are not all in allocated blocks",
                             -2);
+      }
     }
+  }
+  });
   // Dirichlet nodes -> Dirichlet cells (SPGrid_CPU_Solver.cpp:147-192)
   {
     std::unordered_set<uint64_t> dset[3];
@@ -283,18 +351,27 @@ Layout build_layout(const ProblemDesc& p) {
         int X = (x + kB[0]) >> 1, Y = (y + kB[1]) >> 1, Z = (z + kB[2]) >> 1;
         int32_t cb = C.get(X >> 2, Y >> 2, Z >> 2);
         C.cflag[(size_t)cb * 64 + loc(X, Y, Z)] |= (uint16_t)(cf & (F_ACTIVE_CELL | F_DIR_CELL));
-        if (cf & F_ACTIVE_CELL)
-          for (int c = 0; c < 8; ++c) C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
-        if (cf & F_DIR_CELL)
-          for (int c = 1; c < 8; ++c) C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
       }
     }
+    // pages of the coarse cells' nodes: all 8 for an ACTIVE cell, 1..7 for a
+    // DIRICHLET one; marked once per coarse cell (the union over the fine
+    // cells that map to it is the same set)
+    const size_t nbc = C.alloc.size();
+    for (size_t b = 0; b < nbc; ++b)
+      for (int q = 0; q < 64; ++q) {
+        const uint16_t cf = C.cflag[b * 64 + q];
+        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+        const int X = C.bc[3 * b] * 4 + (q >> 4), Y = C.bc[3 * b + 1] * 4 + ((q >> 2) & 3), Z = C.bc[3 * b + 2] * 4 + (q & 3);
+        for (int c = (cf & F_ACTIVE_CELL) ? 0 : 1; c < 8; ++c)
+          C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
+      }
   }
   for (int l = 0; l < out.nlev; ++l) flag_nodes(H[l]);
   for (int l = 0; l < out.nlev; ++l) flag_boundary(H[l]);
 
   // ---- GPU block sets: allocated blocks with an ACTIVE node, Morton order
-  std::vector<std::unordered_map<uint64_t, int32_t>> gmap(out.nlev);
+  std::vector<BlockMap> gmap(out.nlev);
+  for (int l = 0; l < out.nlev; ++l) gmap[l].set_box(box[l].data(), box[l].data() + 3);
   out.lv.resize(out.nlev);
   for (int l = 0; l < out.nlev; ++l) {
     HostLevel& Hl = H[l];
@@ -313,11 +390,10 @@ Layout build_layout(const ProblemDesc& p) {
     D.nb = (int64_t)sel.size();
     D.base.resize(D.nb * 3);
     D.flags.assign(D.nb * 64, 0);
-    gmap[l].reserve(sel.size() * 2);
     for (int64_t g = 0; g < D.nb; ++g) {
       int32_t b = sel[g].second;
       for (int d = 0; d < 3; ++d) D.base[3 * g + d] = Hl.bc[3 * b + d] * 4;
-      gmap[l].emplace(bkey(Hl.bc[3 * b], Hl.bc[3 * b + 1], Hl.bc[3 * b + 2]), (int32_t)g);
+      gmap[l].insert(Hl.bc[3 * b], Hl.bc[3 * b + 1], Hl.bc[3 * b + 2], (int32_t)g);
       for (int q = 0; q < 64; ++q) {
         uint16_t f = Hl.nflag[(size_t)b * 64 + q];
         uint8_t df = 0;
@@ -336,10 +412,7 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
     if (D.nb == 0) throw LayoutError("level " + std::to_string(l) + " has no active nodes", -1);
-    auto gfind = [&](int bx, int by, int bz) {
-      auto it = gmap[l].find(bkey(bx, by, bz));
-      return it == gmap[l].end() ? -1 : it->second;
-    };
+    auto gfind = [&](int bx, int by, int bz) { return gmap[l].find(bx, by, bz); };
     D.nbr.resize(D.nb * 27);
     for (int64_t g = 0; g < D.nb; ++g) {
       int bx = D.base[3 * g] >> 2, by = D.base[3 * g + 1] >> 2, bz = D.base[3 * g + 2] >> 2;
@@ -405,8 +478,7 @@ Layout build_layout(const ProblemDesc& p) {
         int f0[3];
         for (int d = 0; d < 3; ++d) f0[d] = (2 * D.base[3 * g + d] - kB[d]) >> 2;
         for (int s = 0; s < 27; ++s) {
-          auto it = gmap[l - 1].find(bkey(f0[0] + s / 9 - 1, f0[1] + (s / 3) % 3 - 1, f0[2] + s % 3 - 1));
-          D.restrict_src[g * 27 + s] = it == gmap[l - 1].end() ? -1 : it->second;
+          D.restrict_src[g * 27 + s] = gmap[l - 1].find(f0[0] + s / 9 - 1, f0[1] + (s / 3) % 3 - 1, f0[2] + s % 3 - 1);
         }
       }
     }
@@ -416,8 +488,7 @@ Layout build_layout(const ProblemDesc& p) {
         int c0[3];
         for (int d = 0; d < 3; ++d) c0[d] = ((D.base[3 * g + d] + kB[d]) >> 1) >> 2;
         for (int s = 0; s < 8; ++s) {
-          auto it = gmap[l + 1].find(bkey(c0[0] + ((s >> 2) & 1), c0[1] + ((s >> 1) & 1), c0[2] + (s & 1)));
-          D.prolong_src[g * 8 + s] = it == gmap[l + 1].end() ? -1 : it->second;
+          D.prolong_src[g * 8 + s] = gmap[l + 1].find(c0[0] + ((s >> 2) & 1), c0[1] + ((s >> 1) & 1), c0[2] + (s & 1));
         }
       }
     }
@@ -445,8 +516,7 @@ Layout build_layout(const ProblemDesc& p) {
             for (int i = 0; i < nper; ++i)
               for (int j = 0; j < nper; ++j)
                 for (int k = 0; k < nper; ++k) {
-                  auto it = gmap[0].find(bkey(fb[0] + i, fb[1] + j, fb[2] + k));
-                  D.cell_fine_blocks.push_back(it == gmap[0].end() ? -1 : it->second);
+                  D.cell_fine_blocks.push_back(gmap[0].find(fb[0] + i, fb[1] + j, fb[2] + k));
                 }
           }
         }
@@ -457,24 +527,35 @@ Layout build_layout(const ProblemDesc& p) {
     LevelDesc& D = out.lv[0];
     out.E.assign(D.nb * 64, 0.0);
     out.rho.assign(D.nb * 64, 0.0);
-    for (int64_t g = 0; g < D.nb; ++g) {
-      int32_t hb = H0.find(D.base[3 * g] >> 2, D.base[3 * g + 1] >> 2, D.base[3 * g + 2] >> 2);
-      for (int q = 0; q < 64; ++q) {
-        size_t s = (size_t)hb * 64 + q;
-        if (H0.cflag[s] & F_ACTIVE_CELL) { out.E[g * 64 + q] = E0[s]; out.n_active_cells++; }
-        out.rho[g * 64 + q] = R0[s];
+    std::vector<int64_t> ncell(D.nb, 0);
+    parallel_ranges(D.nb, [&](int64_t g0, int64_t g1) {
+      for (int64_t g = g0; g < g1; ++g) {
+        int32_t hb = H0.find(D.base[3 * g] >> 2, D.base[3 * g + 1] >> 2, D.base[3 * g + 2] >> 2);
+        for (int q = 0; q < 64; ++q) {
+          size_t s = (size_t)hb * 64 + q;
+          if (H0.cflag[s] & F_ACTIVE_CELL) { out.E[g * 64 + q] = E0[s]; ncell[g]++; }
+          out.rho[g * 64 + q] = R0[s];
+        }
       }
-    }
+    });
+    for (int64_t c : ncell) out.n_active_cells += c;
     out.nin_blocks = p.nblocks;
     out.in_node_slot.resize(p.nblocks * 64);
-    for (int64_t b = 0; b < p.nblocks; ++b) {
-      const int32_t* bb = p.block_base + 3 * b;
-      for (int l = 0; l < 64; ++l) {
-        int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
-        auto it = gmap[0].find(bkey(x >> 2, y >> 2, z >> 2));
-        out.in_node_slot[b * 64 + l] = it == gmap[0].end() ? -1 : (int32_t)(it->second * 64 + loc(x, y, z));
+    parallel_ranges(p.nblocks, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b) {
+        const int32_t* bb = p.block_base + 3 * b;
+        if (((bb[0] | bb[1] | bb[2]) & 3) == 0) {  // 4-aligned: one level-0 block
+          const int32_t gb = gmap[0].find(bb[0] >> 2, bb[1] >> 2, bb[2] >> 2);
+          for (int l = 0; l < 64; ++l) out.in_node_slot[b * 64 + l] = gb < 0 ? -1 : (int32_t)(gb * 64 + l);
+          continue;
+        }
+        for (int l = 0; l < 64; ++l) {
+          int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
+          const int32_t gb = gmap[0].find(x >> 2, y >> 2, z >> 2);
+          out.in_node_slot[b * 64 + l] = gb < 0 ? -1 : (int32_t)(gb * 64 + loc(x, y, z));
+        }
       }
-    }
+    });
     // u_D (SPGrid_CPU_Solver.cpp:241-246,281-285): written at every listed node
     std::unordered_map<uint64_t, std::vector<int64_t>> in_nodes_off;  // for nodes outside the GPU set
     for (int64_t b = 0; b < p.nblocks; ++b)
@@ -485,9 +566,9 @@ Layout build_layout(const ProblemDesc& p) {
         }
     for (int64_t i = 0; i < p.ndir; ++i) {
       int x = p.dir_coord[3 * i], y = p.dir_coord[3 * i + 1], z = p.dir_coord[3 * i + 2];
-      auto it = gmap[0].find(bkey(x >> 2, y >> 2, z >> 2));
-      if (it != gmap[0].end()) {
-        out.ud_slot.push_back((int32_t)((it->second * 64 + loc(x, y, z)) * 3 + p.dir_axis[i]));
+      const int32_t gb = gmap[0].find(x >> 2, y >> 2, z >> 2);
+      if (gb >= 0) {
+        out.ud_slot.push_back((int32_t)((gb * 64 + loc(x, y, z)) * 3 + p.dir_axis[i]));
         out.ud_value.push_back(p.dir_value[i]);
       } else {
         auto jt = in_nodes_off.find(bkey(x, y, z));

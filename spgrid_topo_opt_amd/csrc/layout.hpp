@@ -63,17 +63,54 @@ struct ProblemDesc {
 // SIMP map (topo_opt/fem_interface.h:343-353)
 double stiffness_mapping(double penalty, double min_stiffness, double min_density, double d);
 
+// Block coordinates -> id: a dense grid over a bounding box (plain array
+// lookups), with a hash map for coordinates outside it or for boxes too
+// large to hold densely.
+struct BlockMap {
+  static constexpr int64_t kMaxDense = int64_t(1) << 26;  // entries (256 MB)
+  int lo[3] = {0, 0, 0}, dim[3] = {0, 0, 0};
+  std::vector<int32_t> grid;
+  std::unordered_map<uint64_t, int32_t> spill;
+  void set_box(const int lo_[3], const int hi_[3]) {  // inclusive bounds
+    int64_t vol = 1;
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = lo_[d];
+      dim[d] = hi_[d] >= lo_[d] ? hi_[d] - lo_[d] + 1 : 0;
+      vol *= dim[d];
+    }
+    if (vol > 0 && vol <= kMaxDense) grid.assign((size_t)vol, -1);
+  }
+  int64_t index(int x, int y, int z) const {
+    if (grid.empty()) return -1;
+    const unsigned dx = (unsigned)(x - lo[0]), dy = (unsigned)(y - lo[1]), dz = (unsigned)(z - lo[2]);
+    if (dx >= (unsigned)dim[0] || dy >= (unsigned)dim[1] || dz >= (unsigned)dim[2]) return -1;
+    return ((int64_t)dx * dim[1] + dy) * dim[2] + dz;
+  }
+  int32_t find(int x, int y, int z) const {
+    const int64_t i = index(x, y, z);
+    if (i >= 0) return grid[(size_t)i];
+    auto it = spill.find(bkey(x, y, z));
+    return it == spill.end() ? -1 : it->second;
+  }
+  // stores id unless the coordinates already have one; returns the stored id
+  int32_t insert(int x, int y, int z, int32_t id) {
+    const int64_t i = index(x, y, z);
+    if (i >= 0) {
+      if (grid[(size_t)i] < 0) grid[(size_t)i] = id;
+      return grid[(size_t)i];
+    }
+    return spill.emplace(bkey(x, y, z), id).first->second;
+  }
+};
+
 // One level on the host while flags are being built.
 struct HostLevel {
-  std::unordered_map<uint64_t, int32_t> map;
+  BlockMap map;
   std::vector<int32_t> bc;       // [nb*3] block coords (block units)
   std::vector<uint8_t> alloc;    // block inside a set page
   std::vector<uint16_t> nflag;   // [nb*64] node flags
   std::vector<uint16_t> cflag;   // [nb*64] cell flags (cell at min corner)
-  int32_t find(int bx, int by, int bz) const {
-    auto it = map.find(bkey(bx, by, bz));
-    return it == map.end() ? -1 : it->second;
-  }
+  int32_t find(int bx, int by, int bz) const { return map.find(bx, by, bz); }
   int32_t get(int bx, int by, int bz);
   void mark_page(int x, int y, int z);  // node coords
   bool node_allocated(int x, int y, int z) const {
