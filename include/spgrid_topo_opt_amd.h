@@ -109,6 +109,12 @@ void spg_destroy(spg_solver *s);
 /* Replace the densities (same block list) and rebuild the coarse operators;
  * flags and topology are kept (only rho changes between optimizer steps).   */
 int spg_update_density(spg_solver *s, const double *density /*host [nblocks*64]*/);
+/* The same update from a device array (input block order, [nblocks*64]):
+ * SIMP map and checks run on the GPU, so nothing crosses the PCIe bus
+ * (fem_interface.h:336-353, SPGrid_CPU_Solver.cpp:112-130).  The array must
+ * be complete when called (the call runs on the solver's own stream); on a
+ * NaN or a change of the active-cell set nothing is modified.               */
+int spg_update_density_device(spg_solver *s, const double *density_dev);
 
 /* Nodal forces (FEMInputs::forces, fem_interface.h:49-57).  Assignment
  * semantics: a later force on the same node replaces an earlier one.        */

@@ -31,6 +31,7 @@ EXPORTS = [
     "spg_destroy", "spg_update_density", "spg_set_forces", "spg_solve", "spg_get_displacement_device",
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
     "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel", "spg_get_info",
+    "spg_update_density_device",
 ]
 
 
@@ -129,6 +130,7 @@ def lib():
         L.spg_bench_apply.argtypes = [vp, C.c_int32, C.c_int32, dp]
         L.spg_bench_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp]
         L.spg_get_info.argtypes = [vp, C.POINTER(Info)]
+        L.spg_update_density_device.argtypes = [vp, vp]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -208,6 +210,10 @@ class Solver:
     def update_density(self, density):
         d = np.ascontiguousarray(density, np.float64).reshape(self.nblocks, 64)
         _check(lib().spg_update_density(self.h, _dp(d)))
+
+    def update_density_device(self, ptr):
+        """densities from a device array (input block order, nblocks*64 float64)"""
+        _check(lib().spg_update_density_device(self.h, C.c_void_p(ptr)))
 
     def solve(self, tolerance=1e-6, inner_tolerance=0.0, max_iterations=500, max_passes=1, precision=0,
               want_output=True, raise_on_error=True, print_residuals=0):

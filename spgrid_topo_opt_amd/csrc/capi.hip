@@ -116,6 +116,14 @@ int spg_update_density(spg_solver* s, const double* density) {
   API_CATCH
 }
 
+int spg_update_density_device(spg_solver* s, const double* density_dev) {
+  API_TRY
+  if (!s || !density_dev) throw SpgError(SPG_E_INVALID, "null argument");
+  s->s->update_density_device(density_dev);
+  return SPG_OK;
+  API_CATCH
+}
+
 int spg_set_forces(spg_solver* s, int64_t n, const int32_t* coord, const double* value) {
   API_TRY
   if (!s || (n > 0 && (!coord || !value))) throw SpgError(SPG_E_INVALID, "null argument");

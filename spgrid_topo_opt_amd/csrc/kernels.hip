@@ -1950,6 +1950,40 @@ __global__ void k_symv_reduce(int n, int S, int64_t stride, const T* __restrict_
   y[i] = a;
 }
 
+// ---------------------------------------------------- density -> SIMP moduli
+// Device form of the density update (SIMP map, topo_opt/fem_interface.h:
+// 336-353, applied as SPGrid_CPU_Solver.cpp:112-130): a read-only check that
+// the active-cell set is unchanged, then the update of rho and E.  A slot
+// holds an active cell exactly when its current E is non-zero.
+__global__ void k_density_check(int64_t n, const int* __restrict__ slot, const double* __restrict__ dens,
+                                const double* __restrict__ E64, int* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = slot[i];
+  if (s < 0) return;
+  const double d = dens[i];
+  int e = 0;
+  if (d != d) e = 1;                                 // NaN
+  else if (E64[s] != 0.0 && !(d > 0)) e = 2;         // would deactivate a cell
+  else if (E64[s] == 0.0 && d > 0) e = 4;            // would activate a cell
+  if (e) atomicOr(err, e);
+}
+__global__ void k_density_update(int64_t n, const int* __restrict__ slot, const double* __restrict__ dens,
+                                 double penalty, double emin, double rmin, double* __restrict__ E64,
+                                 float* __restrict__ E32, double* __restrict__ rho) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = slot[i];
+  if (s < 0) return;
+  const double d = dens[i];
+  rho[s] = d;
+  if (E64[s] != 0.0) {
+    const double e = penalty != 0.0 ? (1.0 - emin) * pow(fmax(rmin, d), penalty) + emin : d;
+    E64[s] = e;
+    E32[s] = (float)e;
+  }
+}
+
 // -------------------------------------------------------------- sensitivity
 // s_e = a_e^T K0 u_e over the 8 nodes of the cell at this slot
 // (objective/minimal_compliance.cpp:86-146; K0 without dx)
@@ -2340,6 +2374,14 @@ void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, 
 void launch_bnd_nslot(hipStream_t st, const LevelDev& L, int* out) {
   const int n = L.bnd_off[8];
   if (n > 0) k_bnd_nslot<<<nblk((int64_t)n * 27), TPB, 0, st>>>(n, L.bnd, L.nbr, out);
+}
+void launch_density_check(hipStream_t st, int64_t n, const int* slot, const double* dens, const double* E64,
+                          int* err) {
+  if (n > 0) k_density_check<<<nblk(n), TPB, 0, st>>>(n, slot, dens, E64, err);
+}
+void launch_density_update(hipStream_t st, int64_t n, const int* slot, const double* dens, double penalty,
+                           double emin, double rmin, double* E64, float* E32, double* rho) {
+  if (n > 0) k_density_update<<<nblk(n), TPB, 0, st>>>(n, slot, dens, penalty, emin, rmin, E64, E32, rho);
 }
 void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v) {
   k_set_diagonal<<<nblk(n), TPB, 0, st>>>(A, n, v);

@@ -150,6 +150,11 @@ void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, 
 void launch_symmetrize_lower(hipStream_t st, double* A, int64_t n);
 void launch_set_diagonal(hipStream_t st, double* A, int64_t n, double v);  // A[i + i n] = v
 void launch_bnd_nslot(hipStream_t st, const LevelDev& L, int* out);  // fills LevelDev::bnslot
+// density update on the device: err |= 1 NaN, 2 deactivation, 4 activation
+void launch_density_check(hipStream_t st, int64_t n, const int* slot, const double* dens, const double* E64,
+                          int* err);
+void launch_density_update(hipStream_t st, int64_t n, const int* slot, const double* dens, double penalty,
+                           double emin, double rmin, double* E64, float* E32, double* rho);
 template <class T>
 void launch_bottom_gather(hipStream_t st, const LevelDev& L, const int* var_of, const T* b, T* x);
 template <class T>

@@ -365,6 +365,25 @@ void Solver::update_density(const double* density) {
   build_operators();
 }
 
+void Solver::update_density_device(const double* density_dev) {
+  // same checks and SIMP map as update_density, on the device; nothing
+  // changes when the check fails
+  const int64_t nin = lay_.nin_blocks * 64;
+  if (!derr_.p) derr_.alloc(16);
+  int* err = derr_.as<int>();
+  HIPCHK(hipMemsetAsync(err, 0, sizeof(int), st_));
+  launch_density_check(st_, nin, in_slot_.as<int>(), density_dev, E64_.as<double>(), err);
+  int herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st_));
+  sync();
+  if (herr & 1) throw SpgError(SPG_E_NAN, "density contains NaN");
+  if (herr & 2) throw SpgError(SPG_E_INVALID, "density update would deactivate a cell (topology change)");
+  if (herr & 4) throw SpgError(SPG_E_INVALID, "density update would activate a cell (topology change)");
+  launch_density_update(st_, nin, in_slot_.as<int>(), density_dev, cfg_.penalty, cfg_.min_stiffness,
+                        cfg_.min_density, E64_.as<double>(), E32_.as<float>(), rho_.as<double>());
+  build_operators();
+}
+
 void Solver::set_forces(int64_t n, const int32_t* coord, const double* value) {
   // ForceOnNode assignment (SPGrid_CPU_Solver.cpp:289-296): later entries win
   if (gmap0_.empty()) {

@@ -67,6 +67,7 @@ class Solver {
 
   void set_forces(int64_t n, const int32_t* coord, const double* value);
   void update_density(const double* density);
+  void update_density_device(const double* density_dev);  // input block order, device
   // returns SPG_OK or error code; fills info
   int solve(const spg_solve_params& prm, spg_info* info);
   int solve_adjoint(const spg_solve_params& prm, const double* g_dev_in3, double* lam_dev_in3, spg_info* info);
@@ -95,6 +96,7 @@ class Solver {
   template <class T> void bottom(const T* b, T* u);
   template <class T> int pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm);
   int print_residuals_ = 0;  // 0 off, 1 stdout, 2 stderr (per PCG iteration)
+  DBuf derr_;                // error flags of the device density check
   template <class T> const T* E() const;
   template <class T> T dxT() const { return (T)cfg_.dx; }
   int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool with_ud);

@@ -41,16 +41,37 @@ class FEM:
                        else 0.0, max_passes=max_passes if precision == 0 else 1, max_iterations=max_iterations)
         self.last_info = None
         self._rho_host = None
+        self._rho_dev = None
+        self._inside_dev = None
 
     def set_density(self, rho: torch.Tensor):
         # cells of the design domain never reach zero density: the caller feeds
         # max(rho, DBL_MIN) so the solver keeps them (spgrid_topo_opt.cpp:1117-1122)
+        if rho.is_cuda:
+            # device path: SIMP map and checks on the GPU, no host round trip
+            r = self._device_density(rho)
+            if self._rho_dev is None or not torch.equal(r, self._rho_dev):
+                torch.cuda.synchronize(rho.device)
+                self.solver.update_density_device(r.data_ptr())
+                self._rho_dev = r
+            return
         r = rho.detach().to(torch.float64).cpu().numpy()
         inside = self.prob.density.reshape(-1) > 0
         r = np.where(inside, np.maximum(r, np.finfo(np.float64).tiny), r)
         if self._rho_host is None or not np.array_equal(r, self._rho_host):
             self.solver.update_density(r.reshape(-1, 64))
             self._rho_host = r.copy()
+
+    def _device_density(self, rho):
+        if self._inside_dev is None or self._inside_dev.device != rho.device:
+            self._inside_dev = torch.as_tensor(self.prob.density.reshape(-1) > 0, device=rho.device)
+        r = rho.detach().to(torch.float64).reshape(-1)
+        return torch.where(self._inside_dev, r.clamp(min=np.finfo(np.float64).tiny), r).contiguous()
+
+    def assume_density(self, rho: torch.Tensor):
+        """Record `rho` (cuda) as the density the solver already holds, e.g.
+        the one it was built with, so the next set_density(rho) is free."""
+        self._rho_dev = self._device_density(rho)
 
     def close(self):
         self.solver.close()
@@ -289,6 +310,8 @@ class TopoOpt:
         torch.cuda.synchronize(dev)
         t1 = time.perf_counter()
         rho_fem = band.fem_density(self.rho, blocks).reshape(-1).clone().requires_grad_(True)
+        if rebuilt:
+            self.fem.assume_density(rho_fem)  # built with exactly these densities
         c = compliance(rho_fem, self.fem)
         c.backward()
         torch.cuda.synchronize(dev)

@@ -388,3 +388,34 @@ def test_print_residuals_and_static_info(box3, capfd):
     for k in ("levels", "active_nodes", "active_cells", "gpu_blocks", "bottom_dofs"):
         assert st[k] == info[k], k
     assert st["hbm_bytes"] > 0
+
+
+def test_device_density_update_matches_host_update():
+    """spg_update_density_device gives the solve of spg_update_density, and
+    a NaN or a topology change raises without modifying the solver"""
+    import torch
+    p = W.box(16, 8, 8, seed=4, mg_levels=3)
+    g = _lib.Solver(p)
+    rng = np.random.default_rng(3)
+    d = np.where(p.density > 0, rng.uniform(0.1, 1.0, p.density.shape), 0.0)
+    g.update_density(d)
+    out_h, ih = g.solve(tolerance=1e-10, precision=1, max_iterations=500)
+    dd = torch.as_tensor(d.reshape(-1), device="cuda")
+    g.update_density(p.density)  # something else in between
+    torch.cuda.synchronize()
+    g.update_density_device(dd.data_ptr())
+    out_d, idv = g.solve(tolerance=1e-10, precision=1, max_iterations=500)
+    # device and host pow may differ in the last bit of E: compare the
+    # displacements (not the roundoff-level residual column) at 1e-10
+    assert idv["iterations"] == ih["iterations"]
+    assert rel(out_d[..., :3], out_h[..., :3]) < 1e-10
+    for bad, code in ((np.nan, _lib.SpgError), (0.0, _lib.SpgError)):
+        x = d.reshape(-1).copy()
+        x[int(np.argmax(d.reshape(-1) > 0))] = bad
+        xd = torch.as_tensor(x, device="cuda")
+        torch.cuda.synchronize()
+        with pytest.raises(code):
+            g.update_density_device(xd.data_ptr())
+    out_a, _ = g.solve(tolerance=1e-10, precision=1, max_iterations=500)
+    assert rel(out_a[..., :3], out_d[..., :3]) == 0  # the failed updates changed nothing
+    g.close()
