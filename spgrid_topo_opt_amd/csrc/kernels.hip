@@ -316,11 +316,11 @@ __device__ inline void dirichlet_inverse(T d[6], uint8_t fl) {
 // u <- u + D^-1 (b - f) on the active, non-Dirichlet components of one node;
 // Dirichlet components are forced to 0 (also on inactive nodes)
 template <class T>
-__device__ inline void gs_node_update(uint8_t fl, T d[6], T f0, T f1, T f2, int blk, int l, T u0, T u1, T u2,
-                                      const T* __restrict__ b, T* __restrict__ u) {
+__device__ inline void gs_node_update_b(uint8_t fl, T d[6], T f0, T f1, T f2, int blk, int l, T u0, T u1, T u2,
+                                        T b0, T b1, T b2, T* __restrict__ u) {
   if (fl & 1) {
     dirichlet_inverse(d, fl);
-    const T r0 = b[vidx(blk, 0, l)] - f0, r1 = b[vidx(blk, 1, l)] - f1, r2 = b[vidx(blk, 2, l)] - f2;
+    const T r0 = b0 - f0, r1 = b1 - f1, r2 = b2 - f2;
     const T n0 = u0 + d[0] * r0 + d[3] * r1 + d[4] * r2;
     const T n1 = u1 + d[3] * r0 + d[1] * r1 + d[5] * r2;
     const T n2 = u2 + d[4] * r0 + d[5] * r1 + d[2] * r2;
@@ -332,6 +332,13 @@ __device__ inline void gs_node_update(uint8_t fl, T d[6], T f0, T f1, T f2, int 
     for (int c = 0; c < 3; ++c)
       if (fl & (2 << c)) u[vidx(blk, c, l)] = T(0);
   }
+}
+// the same with b read here
+template <class T>
+__device__ inline void gs_node_update(uint8_t fl, T d[6], T f0, T f1, T f2, int blk, int l, T u0, T u1, T u2,
+                                      const T* __restrict__ b, T* __restrict__ u) {
+  gs_node_update_b<T>(fl, d, f0, f1, f2, blk, l, u0, u1, u2, b[vidx(blk, 0, l)], b[vidx(blk, 1, l)],
+                      b[vidx(blk, 2, l)], u);
 }
 
 // the k-th node (0..7) of colour c in a 4x4x4 block
@@ -748,6 +755,19 @@ __global__ void __launch_bounds__(256) k_gs0_list_tab(const int* __restrict__ li
                                                       const T* __restrict__ b) {
   const int i = blockIdx.x * 32 + (threadIdx.x >> 3), I = threadIdx.x & 7;
   const bool live = i < n;
+  // the updating lane's own node data do not depend on the row: load first
+  const bool upd = live && I == 0;
+  const int node = upd ? list[i] : 0, nblk0 = node >> 6, nl = node & 63;
+  const uint8_t nfl = upd ? flags[node] : 0;
+  T s0 = 0, s1 = 0, s2 = 0, c0 = 0, c1 = 0, c2 = 0;
+  if (upd) {
+    s0 = u[vidx(nblk0, 0, nl)];
+    s1 = u[vidx(nblk0, 1, nl)];
+    s2 = u[vidx(nblk0, 2, nl)];
+    c0 = b[vidx(nblk0, 0, nl)];
+    c1 = b[vidx(nblk0, 1, nl)];
+    c2 = b[vidx(nblk0, 2, nl)];
+  }
   const int* ns = nslot + (int64_t)(live ? i : 0) * 27;
   const int ix = (I >> 2) & 1, iy = (I >> 1) & 1, iz = I & 1;
   const int cs = live ? ns[(1 - ix) * 9 + (1 - iy) * 3 + (1 - iz)] : -1;  // cell with min corner node - I
@@ -762,11 +782,7 @@ __global__ void __launch_bounds__(256) k_gs0_list_tab(const int* __restrict__ li
   T f0, f1, f2, d[6];
   corner_row<T>(I, e, U, f0, f1, f2);
   octet_row_and_diag<T>(I, e, dx, f0, f1, f2, d);
-  if (I == 0 && live) {
-    const int node = list[i], blk = node >> 6, l = node & 63;
-    gs_node_update<T>(flags[node], d, f0, f1, f2, blk, l, u[vidx(blk, 0, l)], u[vidx(blk, 1, l)],
-                      u[vidx(blk, 2, l)], b, u);
-  }
+  if (upd) gs_node_update_b<T>(nfl, d, f0, f1, f2, nblk0, nl, s0, s1, s2, c0, c1, c2, u);
 }
 // slot (block*64 + local) of the 27 neighbours of every list node, -1 if absent
 __global__ void k_bnd_nslot(int n, const int* __restrict__ list, const int* __restrict__ nbr, int* __restrict__ out) {
