@@ -384,28 +384,22 @@ __device__ __forceinline__ T octet_sum(T x) {
 template <class T, class LoadU>
 __device__ __forceinline__ void corner_row(int I, T e, const LoadU& U, T& f0, T& f1, T& f2) {
   const T sx = (I & 4) ? T(-1) : T(1), sy = (I & 2) ? T(-1) : T(1), sz = (I & 1) ? T(-1) : T(1);
-  T a0 = 0, a1 = 0, a2 = 0;
+  // partial sums per (row v, component w); the signs s_v s_w apply once at the end
+  T a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
   for (int Jp = 0; Jp < 8; ++Jp) {
-    T v0, v1, v2;
-    U(I ^ Jp, v0, v1, v2);
-    v0 *= sx;
-    v1 *= sy;
-    v2 *= sz;
+    T v[3];
+    U(I ^ Jp, v[0], v[1], v[2]);
     const int kb = Jp * 9;
-    a0 = fmaT(KT<T>::k(kb + 0), v0, a0);
-    a0 = fmaT(KT<T>::k(kb + 1), v1, a0);
-    a0 = fmaT(KT<T>::k(kb + 2), v2, a0);
-    a1 = fmaT(KT<T>::k(kb + 3), v0, a1);
-    a1 = fmaT(KT<T>::k(kb + 4), v1, a1);
-    a1 = fmaT(KT<T>::k(kb + 5), v2, a1);
-    a2 = fmaT(KT<T>::k(kb + 6), v0, a2);
-    a2 = fmaT(KT<T>::k(kb + 7), v1, a2);
-    a2 = fmaT(KT<T>::k(kb + 8), v2, a2);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int w = 0; w < 3; ++w) a[r][w] = fmaT(KT<T>::k(kb + 3 * r + w), v[w], a[r][w]);
   }
-  f0 = sx * e * a0;
-  f1 = sy * e * a1;
-  f2 = sz * e * a2;
+  const T ex = sx * e, ey = sy * e, ez = sz * e;
+  f0 = sx * fmaT(ex, a[0][0], fmaT(ey, a[0][1], ez * a[0][2]));
+  f1 = sy * fmaT(ex, a[1][0], fmaT(ey, a[1][1], ez * a[1][2]));
+  f2 = sz * fmaT(ex, a[2][0], fmaT(ey, a[2][1], ez * a[2][2]));
 }
 
 // Octet reduction of the row and of the 3x3 self block
