@@ -183,6 +183,26 @@ int spg_bench_apply(spg_solver *s, int32_t precision, int32_t reps, double *seco
 int spg_bench_kernel(spg_solver *s, int32_t op, int32_t level, int32_t precision, int32_t arg, int32_t reps,
                      double *seconds);
 
+/* ---- optimizer-side kernels on the same 4x4x4 cell blocks (no solver
+ * handle; all arrays device; cell index block*64 + i*16 + j*4 + k; nbr is
+ * [nblocks*27], spoke s = (dx+1)*9 + (dy+1)*3 + (dz+1), -1 where absent). */
+
+/* Sensitivity filter smooth_dc (topo_opt/spgrid_topo_opt.cpp:527-574):
+ *   out(x) = sum_o w(o) dc(x+o) rho(x+o) [x+o inside]
+ *            / (sum_o w(o) [x+o inside] * (rho(x) + 1e-3)),
+ *   w(o) = max(0, r - |o|), |o_i| <= floor(r) <= 3; 0 outside the container. */
+int spg_filter_sensitivity(int64_t nblocks, const int32_t *nbr, const uint8_t *inside, const double *dc,
+                           const double *rho, double filter_size, double *out);
+
+/* One trial of the optimality-criterion bisection
+ * (topo_opt/optimizer/optimality_criterion.cpp:76-118) at multiplier mid:
+ *   new = clamp(clamp(rho sqrt(max(0,-dc)/mid), rho-step, rho+step), min_density, 1)
+ * on the cells with mask != 0, rho elsewhere.  *volume = sum of new over the
+ * masked cells (a deterministic FP64 sum); with out != NULL the new
+ * densities are written there and *change = max |new - rho|.               */
+int spg_oc_trial(int64_t n, const uint8_t *mask, const double *rho, const double *dc, double mid, double step,
+                 double min_density, double *out, double *volume, double *change);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ EXPORTS = [
     "spg_destroy", "spg_update_density", "spg_set_forces", "spg_solve", "spg_get_displacement_device",
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
     "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel", "spg_get_info",
-    "spg_update_density_device",
+    "spg_update_density_device", "spg_filter_sensitivity", "spg_oc_trial",
 ]
 
 
@@ -131,6 +131,8 @@ def lib():
         L.spg_bench_kernel.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp]
         L.spg_get_info.argtypes = [vp, C.POINTER(Info)]
         L.spg_update_density_device.argtypes = [vp, vp]
+        L.spg_filter_sensitivity.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, vp]
+        L.spg_oc_trial.argtypes = [C.c_int64, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, dp, dp]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -306,6 +308,20 @@ class Solver:
         s = C.c_double(0.0)
         _check(lib().spg_bench_kernel(self.h, op, level, precision, arg, reps, C.byref(s)))
         return s.value
+
+
+def filter_sensitivity(nblocks, nbr_ptr, inside_ptr, dc_ptr, rho_ptr, filter_size, out_ptr):
+    """HIP sensitivity filter over device arrays (see spg_filter_sensitivity)"""
+    _check(lib().spg_filter_sensitivity(nblocks, C.c_void_p(nbr_ptr), C.c_void_p(inside_ptr), C.c_void_p(dc_ptr),
+                                        C.c_void_p(rho_ptr), filter_size, C.c_void_p(out_ptr)))
+
+
+def oc_trial(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, out_ptr=None):
+    """one OC trial over device arrays; returns (volume, max change)"""
+    vol, ch = C.c_double(0.0), C.c_double(0.0)
+    _check(lib().spg_oc_trial(n, C.c_void_p(mask_ptr), C.c_void_p(rho_ptr), C.c_void_p(dc_ptr), mid, step,
+                              min_density, C.c_void_p(out_ptr) if out_ptr else None, C.byref(vol), C.byref(ch)))
+    return vol.value, ch.value
 
 
 def fem_solve_bytes(data: bytes, test=False) -> bytes:

@@ -72,6 +72,11 @@ static ProblemDesc make_desc(const spg_config* cfg, int64_t nblocks, const int32
   return p;
 }
 
+namespace spg {
+// error text for entry points defined in other translation units
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace spg
+
 extern "C" {
 
 const char* spg_last_error(void) { return g_err.c_str(); }

@@ -12,7 +12,8 @@ iterate() at :1289-1331): differentiable GPU FEM solves for torch.
   (e.g. a load-node displacement) is differentiable.
 * :func:`filter_sensitivity` / :func:`oc_update` -- the density-weighted cone
   filter (spgrid_topo_opt.cpp:527-574) and the bisection optimality-criterion
-  update (optimizer/optimality_criterion.cpp:19-146) on the GPU with torch ops.
+  update (optimizer/optimality_criterion.cpp:19-146), HIP kernels of the same
+  library (spg_filter_sensitivity, spg_oc_trial).
 
 Densities are tensors of shape (nblocks*64,) in the input block order of the
 Problem (index block*64 + i*16 + j*4 + k), float64 on cuda.
@@ -160,77 +161,52 @@ class CellGrid:
         self.nb = len(bb)
         self.nbr = torch.as_tensor(nbr, device=device)
         self.inside = torch.as_tensor(prob.density.reshape(-1) > 0, device=device)
-
-    def halo(self, v, width=1):
-        """(nb*64,) cell values -> (nb, 4+2w, 4+2w, 4+2w) with neighbour halos (0 outside)."""
-        w = width
-        v = v.reshape(self.nb, 4, 4, 4)
-        padded = torch.cat([v.new_zeros(1, 4, 4, 4), v], 0)  # index 0 = absent block
-        n = 4 + 2 * w
-        out = v.new_zeros(self.nb, n, n, n)
-        for s in range(27):
-            o = (s // 9 - 1, (s // 3) % 3 - 1, s % 3 - 1)
-            src = padded[self.nbr[:, s] + 1]
-            sl_dst, sl_src = [], []
-            for d in range(3):
-                if o[d] < 0:
-                    sl_dst.append(slice(0, w))
-                    sl_src.append(slice(4 - w, 4))
-                elif o[d] > 0:
-                    sl_dst.append(slice(4 + w, 4 + 2 * w))
-                    sl_src.append(slice(0, w))
-                else:
-                    sl_dst.append(slice(w, 4 + w))
-                    sl_src.append(slice(0, 4))
-            out[:, sl_dst[0], sl_dst[1], sl_dst[2]] = src[:, sl_src[0], sl_src[1], sl_src[2]]
-        return out
-
+        self.nbr32 = self.nbr.to(torch.int32).contiguous()
+        self.inside_u8 = self.inside.to(torch.uint8).contiguous()
 
 def filter_sensitivity(grid: CellGrid, dc, rho, filter_size=1.5):
     """dc1(x) = sum_o w(o) dc(x+o) rho(x+o) / (sum_o w(o) [x+o inside] (rho(x)+1e-3)),
-    w(o) = max(0, r - |o|) over |o_i| <= floor(r) (spgrid_topo_opt.cpp:527-574)."""
-    h = int(np.floor(filter_size))
-    inside = grid.inside.to(torch.float64)
-    hv = grid.halo(dc * rho * inside, h)
-    hm = grid.halo(inside, h)
-    num = torch.zeros(grid.nb, 4, 4, 4, dtype=torch.float64, device=dc.device)
-    den = torch.zeros_like(num)
-    for a in range(-h, h + 1):
-        for b in range(-h, h + 1):
-            for c in range(-h, h + 1):
-                w = max(0.0, filter_size - float(np.sqrt(a * a + b * b + c * c)))
-                if w == 0.0:
-                    continue
-                sl = (slice(None), slice(h + a, h + a + 4), slice(h + b, h + b + 4), slice(h + c, h + c + 4))
-                num += w * hv[sl]
-                den += w * hm[sl]
-    out = num.reshape(-1) / (den.reshape(-1) * (rho + 1e-3))
-    return torch.where(grid.inside, out, torch.zeros_like(out))
+    w(o) = max(0, r - |o|) over |o_i| <= floor(r) (spgrid_topo_opt.cpp:527-574),
+    by the HIP kernel spg_filter_sensitivity; 0 outside the container."""
+    if not dc.is_cuda:
+        raise RuntimeError("filter_sensitivity runs on the GPU (HIP kernel); got a CPU tensor")
+    dc = dc.detach().to(torch.float64).contiguous()
+    rho = rho.detach().to(torch.float64).contiguous()
+    out = torch.empty_like(dc)
+    torch.cuda.synchronize(dc.device)
+    _lib.filter_sensitivity(grid.nb, grid.nbr32.data_ptr(), grid.inside_u8.data_ptr(), dc.data_ptr(), rho.data_ptr(),
+                            float(filter_size), out.data_ptr())
+    return out
 
 
 def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None):
     """Bisection on the Lagrange multiplier (optimizer/optimality_criterion.cpp:19-146):
-    new = clamp(old*sqrt(max(0,-dc)/mid), old-step, old+step), clamped to [min_density, 1];
-    the volume sum over the updated cells (`inside`) must equal volume_fraction
-    times the number of container cells (default: the updated cells)."""
+    new = clamp(old*sqrt(max(0,-dc)/mid), old-step, old+step), clamped to [min_density, 1],
+    on the `inside` cells; their volume must equal volume_fraction times the
+    number of container cells (default: the updated cells).  Each trial is the
+    HIP kernel spg_oc_trial; the bisection sequence is the reference's."""
+    if not rho.is_cuda:
+        raise RuntimeError("oc_update runs on the GPU (HIP kernel); got a CPU tensor")
     n = inside.sum().item() if container_cells is None else container_cells
     target = volume_fraction * n
-    neg = torch.clamp(-dc, min=0.0)
-
-    def trial(mid):
-        new = torch.clamp(rho * torch.sqrt(neg / mid), rho - step_limit, rho + step_limit)
-        new = torch.clamp(new, min_density, 1.0)
-        return torch.where(inside, new, rho)
-
+    rho = rho.detach().to(torch.float64).contiguous()
+    dc = dc.detach().to(torch.float64).contiguous()
+    mask = inside.to(torch.uint8).contiguous()
+    torch.cuda.synchronize(rho.device)
     lower, upper = 1e-20, 1e20
+    mid = 0.5 * (lower + upper)
     while lower * (1 + 1e-7) < upper:
         mid = 0.5 * (lower + upper)
-        if trial(mid)[inside].sum().item() > target:
+        vol, _ = _lib.oc_trial(rho.numel(), mask.data_ptr(), rho.data_ptr(), dc.data_ptr(), mid, step_limit,
+                               min_density)
+        if vol > target:
             lower = mid
         else:
             upper = mid
-    new = trial(mid)
-    return new, (new - rho).abs().max().item()
+    new = torch.empty_like(rho)
+    _, change = _lib.oc_trial(rho.numel(), mask.data_ptr(), rho.data_ptr(), dc.data_ptr(), mid, step_limit,
+                              min_density, new.data_ptr())
+    return new, change
 
 
 class TopoOpt:
