@@ -592,18 +592,18 @@ __device__ __forceinline__ T quad_sum(T x) {
 // corner_row), the quad sums, and lane q = 0 updates the node.  A 256-thread
 // workgroup stages the 5^3 halos of its 8 blocks.  Same race argument as
 // k_gs0_colour.
-static constexpr int Q_BLOCKS = 8;
-template <class T>
-__global__ void __launch_bounds__(256) k_gs0_colour4(int nb, int colour, const int* __restrict__ nbr,
-                                                     const uint8_t* __restrict__ flags, const T* __restrict__ E,
-                                                     T dx, T* __restrict__ u, const T* __restrict__ b) {
-  __shared__ T hu[Q_BLOCKS][3][128];
-  __shared__ int ht[Q_BLOCKS][27];
+template <class T, int QB>
+__global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, const int* __restrict__ nbr,
+                                                         const uint8_t* __restrict__ flags, const T* __restrict__ E,
+                                                         T dx, T* __restrict__ u, const T* __restrict__ b) {
+  static_assert(QB >= 4, "the 125-entry halo table is filled by one pass of the workgroup");
+  __shared__ T hu[QB][3][128];
+  __shared__ int ht[QB][27];
   __shared__ int htab[128];
   const int tid = threadIdx.x;
-  const int blk0 = xcd_chunk(blockIdx.x, gridDim.x) * Q_BLOCKS;
+  const int blk0 = xcd_chunk(blockIdx.x, gridDim.x) * QB;
   const int px = (colour >> 2) & 1, py = (colour >> 1) & 1, pz = colour & 1;
-  if (tid < Q_BLOCKS * 27) {
+  if (tid < QB * 27) {
     const int bl = tid / 27, sp = tid - bl * 27, blk = blk0 + bl;
     ht[bl][sp] = blk < nb ? (sp == 13 ? blk : nbr[blk * 27 + sp]) : -1;
   }
@@ -2225,7 +2225,16 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
   if (kind == 2) {
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   } else if (kind == 3) {
-    k_gs0_colour4<T><<<(L.nb + Q_BLOCKS - 1) / Q_BLOCKS, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+    // SPG_GS_QB blocks per workgroup (4, 8 or 16); 256^3 shell, FP32 / FP64
+    // sweep: 4 -> 318-327 / 478-480 us, 8 -> 322 / 483 us, 16 -> 349 / 552 us
+    const char* q = std::getenv("SPG_GS_QB");
+    const int qb = q ? std::atoi(q) : 8;
+    if (qb == 4)
+      k_gs0_colour4<T, 4><<<(L.nb + 3) / 4, 128, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+    else if (qb == 16)
+      k_gs0_colour4<T, 16><<<(L.nb + 15) / 16, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+    else
+      k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   } else {
     const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);

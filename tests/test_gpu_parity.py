@@ -111,13 +111,16 @@ def test_gs_sweep_level0(blocky, prec, tol, forward, boundary):
     assert rel(got, want) < tol
 
 
-@pytest.mark.parametrize("kind", ["1", "2", "3"])
+@pytest.mark.parametrize("kind", ["1", "2", "3", "3:4", "3:16"])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
 def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
     """every level-0 colour kernel (SPG_GS_KERNEL=1: eight lanes per node;
-    2: a cell corner per wavefront; 3: four lanes per node)"""
+    2: a cell corner per wavefront; 3: four lanes per node, with
+    SPG_GS_QB blocks per workgroup)"""
     P = blocky
+    kind, _, qb = kind.partition(":")
     monkeypatch.setenv("SPG_GS_KERNEL", kind)
+    monkeypatch.setenv("SPG_GS_QB", qb or "8")
     rng = np.random.default_rng(6)
     n = len(P.map[0][0])
     gf = P.map[0][1]
