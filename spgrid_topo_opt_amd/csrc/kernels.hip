@@ -997,12 +997,17 @@ __global__ void k_bnd_coef_st(int n, const int* __restrict__ list, const double*
 }
 
 // b~_i = b_i - sum over spokes to nodes outside the boundary set (bt is
-// padded to 4 entries per node)
+// padded to 4 entries per node).  With `ub` given it also copies the
+// boundary iterate to ub ([3][n+1], zero sentinel at n) and clears the grid
+// barrier counter `ctr` of the cooperative phase.
 template <class T>
 __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ list, const int* __restrict__ bslot,
                                                  const T* __restrict__ A, const T* __restrict__ u,
-                                                 const T* __restrict__ b, T* __restrict__ bt) {
+                                                 const T* __restrict__ b, T* __restrict__ bt, T* __restrict__ ub = nullptr,
+                                                 unsigned* __restrict__ ctr = nullptr) {
   const int i = blockIdx.x * 32 + (threadIdx.x >> 3), g = threadIdx.x & 7;
+  if (ub && blockIdx.x == 0 && threadIdx.x < 3) ub[threadIdx.x * (n + 1) + n] = T(0);
+  if (ctr && blockIdx.x == 0 && threadIdx.x == 0) *ctr = 0u;
   const bool live = i < n;
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
@@ -1031,6 +1036,9 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
     bt[i * 4 + 1] = b[vidx(blk, 1, l)] - a1;
     bt[i * 4 + 2] = b[vidx(blk, 2, l)] - a2;
     bt[i * 4 + 3] = T(0);
+    if (ub)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ub[c * (n + 1) + i] = u[vidx(blk, c, l)];
   }
 }
 
@@ -1529,6 +1537,174 @@ __global__ void __launch_bounds__(B0_THREADS) k_bnd0_sweeps(int n, BndOff L, Col
     const int node = list[i], blk = node >> 6, l = node & 63;
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) u[vidx(blk, cc, l)] = ub[cc * n1 + i];
+  }
+}
+
+// Level-0 boundary phase over several CUs: a cooperative launch of G
+// workgroups (all resident at once), each of 8 waves; wave I computes cell
+// corner I's row for 64 nodes of a colour, the corner rows meet in LDS and
+// wave 0 updates the nodes.  The boundary iterate lives in `ub` ([3][n+1],
+// entry n a zero sentinel, filled by k_bnd_rhs); colour steps are separated
+// by a grid barrier.  Each node's update in the last sweep also goes to u.
+// A barrier wait longer than `limit` wall-clock ticks sets coop[1] and ends
+// the kernel, so every wave exits whatever happens.
+static constexpr int BC_THREADS = 512;
+static constexpr int BC_MAX_WG = 32;
+// per-lane tables of one node group: the corner's modulus and node indices,
+// and (wave 0) b~, the inverse self block, the node and its flags
+template <class T>
+struct CoopFetch {
+  T e, r[3], di[6];
+  int cw[4];
+  int node, i;
+  uint8_t fl;
+  bool live;
+};
+template <class T>
+__global__ void __launch_bounds__(BC_THREADS) k_bnd0_coop(int n, BndOff L, ColourOrder ord, int sweeps,
+                                                          const int* __restrict__ list,
+                                                          const uint8_t* __restrict__ flags,
+                                                          const T* __restrict__ bE,
+                                                          const int16_t* __restrict__ cnode,
+                                                          const T* __restrict__ bD, const T* __restrict__ bt, T dx,
+                                                          T* ub, T* __restrict__ u, unsigned* coop,
+                                                          unsigned long long limit) {
+  __shared__ T P[8][3][64];
+  __shared__ int bad;
+  const int n1 = n + 1;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int I = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int G = gridDim.x;
+  const int steps = sweeps * 8;
+  unsigned* ctr = coop;
+  int* err = reinterpret_cast<int*>(coop + 1);
+  // everything but the iterate: loaded ahead, the next step's first group
+  // while the grid barrier is pending
+  auto fetch = [&](CoopFetch<T>& F, int st, int q) {
+    const int c = ord.at(st & 7), o0 = L.off[c], nc = L.off[c + 1] - o0;
+    const int j = q * 64 + lane;
+    F.live = j < nc;
+    F.i = o0 + (F.live ? j : 0);
+    F.e = T(0);
+    F.cw[0] = F.cw[1] = F.cw[2] = F.cw[3] = -1;
+    if (F.live) {
+      F.e = bE[I * n + F.i];
+      const int4 a = *reinterpret_cast<const int4*>(cnode + ((size_t)I * n + F.i) * 8);
+      F.cw[0] = a.x;
+      F.cw[1] = a.y;
+      F.cw[2] = a.z;
+      F.cw[3] = a.w;
+    }
+#pragma unroll
+    for (int v = 0; v < 3; ++v) F.r[v] = T(0);
+#pragma unroll
+    for (int v = 0; v < 6; ++v) F.di[v] = T(0);
+    F.node = 0;
+    F.fl = 0;
+    if (I == 0 && F.live) {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) F.r[v] = bt[F.i * 4 + v];
+#pragma unroll
+      for (int v = 0; v < 6; ++v) F.di[v] = bD[F.i * 8 + v];
+      F.node = list[F.i];
+      F.fl = flags[F.node];
+    }
+  };
+  CoopFetch<T> F;
+  fetch(F, 0, blockIdx.x);
+  for (int st = 0; st < steps; ++st) {
+    const int c = ord.at(st & 7);
+    const bool last = st >= steps - 8;
+    const int nc = L.off[c + 1] - L.off[c];
+    for (int q = blockIdx.x; q * 64 < nc; q += G) {
+      if (q != (int)blockIdx.x) fetch(F, st, q);
+      T a0 = 0, a1 = 0, a2 = 0, x[3];
+#pragma unroll
+      for (int J = 0; J < 8; ++J) {
+        const int wv = F.cw[J >> 1];
+        int k = (J & 1) ? (wv >> 16) : (int)(short)(wv & 0xffff);
+        k = k < 0 ? n : k;
+        const T v0 = ub[k], v1 = ub[n1 + k], v2 = ub[2 * n1 + k];
+        if (J == 0) {  // for corner 0, node 0 of the cell is the node itself
+          x[0] = v0;
+          x[1] = v1;
+          x[2] = v2;
+        }
+        const int kb = (I * 8 + J) * 9;
+        a0 = fmaT(KT<T>::k(kb + 0), v0, a0);
+        a0 = fmaT(KT<T>::k(kb + 1), v1, a0);
+        a0 = fmaT(KT<T>::k(kb + 2), v2, a0);
+        a1 = fmaT(KT<T>::k(kb + 3), v0, a1);
+        a1 = fmaT(KT<T>::k(kb + 4), v1, a1);
+        a1 = fmaT(KT<T>::k(kb + 5), v2, a1);
+        a2 = fmaT(KT<T>::k(kb + 6), v0, a2);
+        a2 = fmaT(KT<T>::k(kb + 7), v1, a2);
+        a2 = fmaT(KT<T>::k(kb + 8), v2, a2);
+      }
+      P[I][0][lane] = F.e * a0;
+      P[I][1][lane] = F.e * a1;
+      P[I][2][lane] = F.e * a2;
+      __syncthreads();
+      if (I == 0 && F.live) {
+        const int i = F.i;
+        T f[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          T acc = 0;
+#pragma unroll
+          for (int cc = 0; cc < 8; ++cc) acc += P[cc][v][lane];
+          f[v] = dx * acc;
+        }
+        const uint8_t fl = F.fl;
+        if (fl & 1) {
+          const T r0 = F.r[0] - f[0], r1 = F.r[1] - f[1], r2 = F.r[2] - f[2];
+          const T u0 = x[0] + F.di[0] * r0 + F.di[3] * r1 + F.di[4] * r2;
+          const T u1 = x[1] + F.di[3] * r0 + F.di[1] * r1 + F.di[5] * r2;
+          const T u2 = x[2] + F.di[4] * r0 + F.di[5] * r1 + F.di[2] * r2;
+          x[0] = (fl & 2) ? T(0) : u0;
+          x[1] = (fl & 4) ? T(0) : u1;
+          x[2] = (fl & 8) ? T(0) : u2;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 3; ++v)
+            if (fl & (2 << v)) x[v] = T(0);
+        }
+#pragma unroll
+        for (int v = 0; v < 3; ++v) ub[v * n1 + i] = x[v];
+        if (last) {
+          const int blk = F.node >> 6, l = F.node & 63;
+#pragma unroll
+          for (int v = 0; v < 3; ++v) u[vidx(blk, v, l)] = x[v];
+        }
+      }
+      __syncthreads();  // P is reused by the next group
+    }
+    if (st + 1 == steps) break;
+    // Grid barrier.  Thread 0 (of wave 0, the only wave that stores to
+    // global memory) publishes the step with a release add, the workgroup
+    // then loads the next step's tables, and thread 0 waits for all G
+    // arrivals; a wait longer than `limit` wall-clock ticks sets the error
+    // word and ends the kernel, so every wave exits.
+    const unsigned target = (unsigned)(G * (st + 1));
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    fetch(F, st + 1, blockIdx.x);
+    if (tid == 0) {
+      int b = 0;
+      const unsigned long long t0 = wall_clock64();
+      while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (wall_clock64() - t0 > limit) {
+          __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          b = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __threadfence();
+      bad = b;
+    }
+    __syncthreads();
+    if (bad) return;
   }
 }
 
@@ -2111,11 +2287,12 @@ static BndLists bnd_lists(const LevelDev& L) {
 // a boundary phase runs in one workgroup while no colour list is longer than this
 static constexpr int kFusedBoundaryMax = 4096;
 // SPG_BOUNDARY_PATH selects the boundary smoother for tests: 0 (default)
-// automatic (level 0: one launch per colour step; explicit levels: the
-// dense map, else the LDS coefficient-row system), 1 the single-workgroup
-// global-memory kernel, 2 one launch per colour step, 3 the coefficient-row
-// LDS system also on level 0, 4 the matrix-free single-workgroup level-0
-// phase.
+// automatic (level 0: the cooperative matrix-free phase, else one launch per
+// colour step; explicit levels: the dense map, else the LDS coefficient-row
+// system), 1 the single-workgroup global-memory kernel, 2 one launch per
+// colour step, 3 the coefficient-row LDS system also on level 0, 4 the
+// matrix-free single-workgroup level-0 phase, 5 the cooperative level-0
+// phase (automatic on the explicit levels).
 static int boundary_path() {
   const char* e = std::getenv("SPG_BOUNDARY_PATH");
   return e ? std::atoi(e) : 0;
@@ -2191,6 +2368,50 @@ static bool launch_bnd0_phase(hipStream_t st, const LevelDev& L, T dx, T* u, con
                                                L.b0node, bD, bt, dx, u);
   return true;
 }
+// level-0 boundary phase as one cooperative launch over several CUs; false
+// if the tables are absent or the launch is refused (the caller then runs
+// another path; `loud` reports a refused launch on stderr).  SPG_COOP_WG caps the workgroup count (tests use it to make
+// a workgroup take several node groups per colour step).
+template <class T>
+static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, const T* b, int sweeps, bool forward,
+                             bool loud) {
+  const int n = L.bnd_off[8];
+  const T* A = BndPtr<T>::A(L);
+  T* bt = BndPtr<T>::bt(L);
+  const T* bE = Bnd0Ptr<T>::E(L);
+  const T* bD = Bnd0Ptr<T>::D(L);
+  if (!A || !bt || !bE || !bD || !L.b0node || !L.bslot || !L.b0ub || !L.coop) return false;
+  int groups = 1;
+  for (int c = 0; c < 8; ++c) groups = std::max(groups, (L.bnd_off[c + 1] - L.bnd_off[c] + 63) / 64);
+  int G = std::min(groups, BC_MAX_WG);
+  if (const char* e = std::getenv("SPG_COOP_WG"))
+    if (std::atoi(e) > 0) G = std::min(G, std::atoi(e));
+  static const unsigned long long limit = [] {
+    int dev = 0, khz = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    return (unsigned long long)khz * 50;  // 50 ms per barrier wait
+  }();
+  T* ub = reinterpret_cast<T*>(L.b0ub);
+  k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt, ub, L.coop);
+  BndOff o;
+  for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
+  ColourOrder ord = colour_order(true, forward);
+  const int16_t* cn = L.b0node;
+  const int* list = L.bnd;
+  const uint8_t* flags = L.flags;
+  unsigned* coop = L.coop;
+  unsigned long long lim = limit;
+  int nn = n, sw = sweeps;
+  void* args[] = {&nn, &o, &ord, &sw, &list, &flags, &bE, &cn, &bD, &bt, &dx, &ub, &u, &coop, &lim};
+  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_bnd0_coop<T>, dim3(G), dim3(BC_THREADS), args, 0, st);
+  if (e == hipSuccess) return true;
+  (void)hipGetLastError();
+  if (loud)
+    std::fprintf(stderr, "spgrid_topo_opt_amd: cooperative boundary phase not launched (%s); using the list path\n",
+               hipGetErrorString(e));
+  return false;
+}
 void launch_bnd0_setup(hipStream_t st, const LevelDev& L, const int* cslot, const double* E, double dx, double* bE,
                        double* bD) {
   const int n = L.bnd_off[8];
@@ -2246,12 +2467,16 @@ void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T*
   if (sweeps <= 0 || L.bnd_off[8] == 0) return;
   int path = boundary_path();
   // Automatic choice, from measurement on the 256^3 shell (3655 boundary
-  // nodes, 3 sweeps): the single-workgroup phase is one CU's VALU work, the
-  // per-colour list launches spread each colour step over many CUs.  FP32:
-  // 154 vs 144 us; FP64: 235 vs 162 us.  Larger sets favour the lists more.
-  if (path == 0) path = 2;
-  if ((path == 0 || path == 4) && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
-  if ((path == 0 || path == 3) && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
+  // nodes, 3 sweeps), FP32 / FP64 per phase: cooperative launch 123 / 136
+  // us, per-colour list launches 145 / 168 us, single workgroup 154 / 235
+  // us (one CU's VALU work).  The lists remain for sets beyond the int16
+  // node tables and for a refused cooperative launch.
+  if (path == 0 || path == 5) {
+    if (launch_bnd0_coop<T>(st, L, dx, u, b, sweeps, forward, path == 5)) return;
+    path = 2;
+  }
+  if (path == 4 && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;
+  if (path == 3 && launch_bnd_phase<T>(st, L, u, b, sweeps, true, forward)) return;
   const ColourOrder ord = colour_order(true, forward);
   if (path != 2 && L.bnd_max <= kFusedBoundaryMax) {
     k_gs0_boundary<T><<<1, BND_THREADS, 0, st>>>(bnd_lists(L), ord, sweeps, L.nbr, L.flags, E, dx, u, b);
@@ -2290,7 +2515,7 @@ struct BmapPtr<double> {
 template <class T>
 void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int sweeps, bool forward) {
   if (sweeps <= 0 || L.bnd_off[8] == 0) return;
-  const int path = boundary_path();
+  const int path = boundary_path() == 5 ? 0 : boundary_path();  // 5 selects a level-0 phase only
   const T* M = BmapPtr<T>::M(L, forward);
   if (path == 0 && M && sweeps == L.bmap_sweeps) {
     const int n = L.bnd_off[8];

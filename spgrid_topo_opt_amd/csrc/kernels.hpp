@@ -37,6 +37,10 @@ struct LevelDev {
   const double* b0E64 = nullptr;
   const float* b0D32 = nullptr;
   const double* b0D64 = nullptr;
+  // cooperative level-0 phase: boundary iterate [3*(n+1)] (8-byte entries,
+  // either precision) and {barrier counter, error word}
+  void* b0ub = nullptr;
+  unsigned* coop = nullptr;
   // explicit levels: dense maps [G H] of one boundary phase (forward,
   // backward colour order) for `bmap_sweeps` sweeps, and their vector
   // scratch ([6n] input then [3n] output)

@@ -103,6 +103,14 @@ Solver::~Solver() {
 
 void Solver::sync() { HIPCHK(hipStreamSynchronize(st_)); }
 
+void Solver::check_coop() {
+  if (lv_.empty() || !lv_[0].coop.p) return;
+  unsigned w[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(w, lv_[0].coop.p, sizeof(w), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  if (w[1]) throw SpgError(SPG_E_HIP, "level-0 boundary phase: grid barrier wait timed out");
+}
+
 void Solver::upload_layout() {
   const int L = lay_.nlev;
   lv_.clear();
@@ -147,6 +155,11 @@ void Solver::upload_layout() {
       R.d.b0E32 = R.b0E32.as<float>();
       R.d.b0D64 = R.b0D64.as<double>();
       R.d.b0D32 = R.b0D32.as<float>();
+      R.b0ub.alloc((bnd.size() + 1) * 3 * 8);
+      R.coop.alloc(2 * sizeof(unsigned));
+      HIPCHK(hipMemset(R.coop.p, 0, 2 * sizeof(unsigned)));
+      R.d.b0ub = R.b0ub.p;
+      R.d.coop = R.coop.as<unsigned>();
     }
     R.d.bci = R.bci.as<int>();
     R.d.bslot = R.bslot.as<int>();
@@ -661,6 +674,7 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
     info->r_l2 = l2;
     info->solve_seconds = now_s() - t0;
   }
+  check_coop();
   if (rc == SPG_OK && !conv) rc = SPG_E_NOT_CONVERGED;
   return rc;
 }
@@ -890,6 +904,7 @@ void Solver::run_kernel(int op, int level, int precision, int forward, const dou
   };
   if (precision == 1) body(double());
   else body(float());
+  check_coop();
 }
 
 void Solver::level_stencils(int level, double* out) {
@@ -1000,6 +1015,7 @@ double Solver::bench_kernel(int op, int level, int precision, int arg, int reps)
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  check_coop();
   return (double)ms * 1e-3 / reps;
 }
 

@@ -50,6 +50,7 @@ struct LevelRT {
   DBuf st64, st32, cell_row, cfb, cgi;
   DBuf bci, bslot, bA64, bA32, bt64, bt32;  // boundary system
   DBuf b0cslot, b0node, b0E64, b0E32, b0D64, b0D32;  // level-0 matrix-free boundary system
+  DBuf b0ub, coop;  // cooperative level-0 phase: iterate and {barrier counter, error word}
   DBuf bmap64[2], bmap32[2], bmx64, bmx32;           // dense boundary maps (explicit levels)
   DBuf bnslot;                                       // level 0: neighbour slots of boundary nodes
   DBuf u32, b32, r32, u64, b64, r64;
@@ -105,6 +106,8 @@ class Solver {
   void upload_layout();
   double read_scalar(const double* dev);
   void sync();
+  // throws if a cooperative boundary phase reported a barrier time-out
+  void check_coop();
 
   spg_config cfg_;
   Layout lay_;

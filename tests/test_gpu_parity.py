@@ -133,13 +133,13 @@ def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
         assert rel(got, want) < tol
 
 
-@pytest.mark.parametrize("path", ["1", "2", "3", "4"])
+@pytest.mark.parametrize("path", ["1", "2", "3", "4", "5"])
 def test_boundary_fallback_paths(blocky, path, monkeypatch):
     """every boundary smoother (1: one workgroup from global memory; 2:
-    per-colour list launches, the default for FP64 and large sets on level 0;
-    3: coefficient rows in LDS on level 0 too; 4: the matrix-free
-    one-workgroup level-0 phase in either precision): same result as the
-    oracle on every smoothed level"""
+    per-colour list launches on level 0; 3: coefficient rows in LDS on level
+    0 too; 4: the matrix-free one-workgroup level-0 phase; 5: the matrix-free
+    level-0 phase as one cooperative launch): same result as the oracle on
+    every smoothed level"""
     P = blocky
     monkeypatch.setenv("SPG_BOUNDARY_PATH", path)
     rng = np.random.default_rng(11)
@@ -155,7 +155,7 @@ def test_boundary_fallback_paths(blocky, path, monkeypatch):
             assert rel(got, want) < tol, (lv, prec)
 
 
-@pytest.mark.parametrize("path", ["0", "1"])
+@pytest.mark.parametrize("path", ["0", "1", "5"])
 def test_boundary_phase_multi_sweep(blocky, path, monkeypatch):
     """a whole boundary phase (as many sweeps as the V-cycle runs) on every
     smoothed level; on the explicit levels path 0 runs it as one dense map
@@ -264,6 +264,39 @@ def _solution_error(P, out4, u_o):
     want = P.o.pack(u_o)[..., :3]
     got = out4[..., :3]
     return np.linalg.norm(got - want) / np.linalg.norm(want)
+
+
+@pytest.fixture(scope="module")
+def wide_face():
+    """a 64x32x32 box clamped on a whole face: about 4,400 level-0 boundary
+    nodes, 9 groups of 64 nodes per colour"""
+    return _lib.Solver(W.box(64, 32, 32, seed=3, mg_levels=3))
+
+
+@pytest.mark.parametrize("wg", ["", "1", "2", "4"])
+def test_cooperative_boundary_phase_matches_list_path(wide_face, wg, monkeypatch, capfd):
+    """the cooperative level-0 phase (grid barriers between colour steps,
+    SPG_COOP_WG workgroups, each taking several node groups when capped)
+    against the per-colour list launches, whole phases in both colour
+    orders; FP64 to 1e-12, FP32 to 1e-5 relative (only the summation order
+    differs)"""
+    s = wide_face
+    monkeypatch.setenv("SPG_COOP_WG", wg)
+    gx, gf = s.level_nodes(0)
+    n = len(gx)
+    rng = np.random.default_rng(21)
+    u = rng.standard_normal((n, 3)) * 1e-3
+    u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    b = rng.standard_normal((n, 3))
+    for fwd in (True, False):
+        for prec, tol in ((1, 1e-12), (0, 1e-5)):
+            monkeypatch.setenv("SPG_BOUNDARY_PATH", "2")
+            want = s.kernel(_lib.OP_GS_BOUNDARY, 0, u, b, precision=prec, forward=fwd, sweeps=3)
+            monkeypatch.setenv("SPG_BOUNDARY_PATH", "5")
+            got = s.kernel(_lib.OP_GS_BOUNDARY, 0, u, b, precision=prec, forward=fwd, sweeps=3)
+            assert rel(got, want) < tol, (fwd, prec)
+            assert np.abs(got - u).max() > 0
+    assert "not launched" not in capfd.readouterr().err
 
 
 @pytest.mark.parametrize("levels", [0, 1, 3, 4])
