@@ -28,6 +28,7 @@ METRIC = "MG-PCG solve time to 1e-6 rel. residual, mixed vs FP64; MDOF/s stiffne
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE_FP32 = 28.0  # u 12 + f 12 + E 4 (algorithmic, per active node)
 BYTES_PER_NODE_FP64 = 56.0
+GS_SWEEP_BYTES_PER_NODE_FP32 = 40.0  # one 8-colour sweep: u read + write 24, b 12, E 4 (algorithmic)
 
 
 def log(*a):
@@ -97,6 +98,7 @@ def main():
     os.environ["SPG_ACC64_MODES"] = "0"
     t_ap32f = s.bench_apply(0, args.apply_reps)
     del os.environ["SPG_ACC64_MODES"]
+    t_gs32 = s.bench_kernel(_lib.OP_GS_SWEEP, 0, 0, 0, 20)  # 8 colour launches
     t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
     t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
     log(f"[bench] V-cycle fp32 {t_vc32 * 1e3:.3f} ms, fp64 {t_vc64 * 1e3:.3f} ms")
@@ -195,6 +197,17 @@ def main():
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
             "traffic": measured_traffic("k_apply_lds<float, double, 2>", nn),
+        },
+        "roofline_gs_sweep": {
+            "bound": "hbm",
+            "kernel": "k_gs0_colour4<float, 8> (level-0 Gauss-Seidel, 8 colour launches per sweep)",
+            "achieved": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9 / HBM_PEAK_GBS,
+            "us_per_sweep": t_gs32 * 1e6,
+            "traffic_per_sweep": (lambda t: None if t is None else 8 * t)(
+                measured_traffic("k_gs0_colour4<float, 8>", nn)),
         },
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_reference": acc,

@@ -201,11 +201,14 @@ static constexpr int AP_WAVES = 4;
 // T: storage type of the vectors and moduli; A: arithmetic type of the row
 // sums (FP32 vectors may be summed in FP64, which gfx950 issues at the same
 // rate as FP32 on the vector ALUs)
+// MODE 0: out = K in; 1: out = P (b - K in); 2: out = P K in; 3: as 2, and
+// each block's partial sum of in . out goes to dpart[block].
 template <class T, class A, int MODE>
 __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* __restrict__ nbr,
                                                              const uint8_t* __restrict__ flags,
                                                              const T* __restrict__ E, T dx, const T* __restrict__ in,
-                                                             const T* __restrict__ b, T* __restrict__ out) {
+                                                             const T* __restrict__ b, T* __restrict__ out,
+                                                             double* __restrict__ dpart = nullptr) {
   // the halo stays in the input type; values widen to A when read
   __shared__ T hu[AP_WAVES][3][216];
   __shared__ T he[AP_WAVES][128];
@@ -281,17 +284,26 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
     f1 = fmaT(e, t1, f1);
     f2 = fmaT(e, t2, f2);
   }
-  if (!live) return;
+  if (!live) return;  // wave-uniform: a wave owns one block
   const T f[3] = {(T)((A)dx * f0), (T)((A)dx * f1), (T)((A)dx * f2)};
   if (MODE == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[vidx(blk, c, lane)] = f[c];
   } else {
     const uint8_t fl = flags[blk * 64 + lane];
+    double d = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T v = MODE == 2 ? f[c] : b[vidx(blk, c, lane)] - f[c];
-      out[vidx(blk, c, lane)] = (fl & (2 << c)) ? T(0) : v;
+      const T v = MODE >= 2 ? f[c] : b[vidx(blk, c, lane)] - f[c];
+      const T o = (fl & (2 << c)) ? T(0) : v;
+      out[vidx(blk, c, lane)] = o;
+      if (MODE == 3) d += (double)hu[w][c][(li + 1) * 36 + (lj + 1) * 6 + (lk + 1)] * (double)o;
+    }
+    if (MODE == 3) {
+      // one partial per wavefront (= per block)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
+      if (lane == 0) dpart[blk] = d;
     }
   }
 }
@@ -1900,6 +1912,37 @@ __global__ void __launch_bounds__(RED_T) k_final(const double* part, int np, dou
   if (threadIdx.x == 0) *out = s;
 }
 
+__global__ void __launch_bounds__(RED_T) k_sum_part(const double* a, int64_t n, double* part) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i];
+  s = block_reduce<0>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// sum of np partials in a fixed order by one 1024-thread workgroup
+__global__ void __launch_bounds__(1024) k_sum_wide(const double* __restrict__ part, int np, double* out) {
+  __shared__ double sh[16];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * 1024 < np; i += 4 * 1024) {
+    a0 += part[i];
+    a1 += part[i + 1024];
+    a2 += part[i + 2048];
+    a3 += part[i + 3072];
+  }
+  for (; i < np; i += 1024) a0 += part[i];
+  double v = (a0 + a1) + (a2 + a3);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sh[0];
+    for (int q = 1; q < 16; ++q) t += sh[q];
+    *out = t;
+  }
+}
+
 // ----------------------------------------------------------------------- PCG
 template <class T>
 __global__ void k_cg_direction(T* s, const T* z, const CgScalars* sc, int64_t n) {
@@ -2273,6 +2316,25 @@ void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* 
   if (sizeof(T) == 4 && ((acc64_modes() >> mode) & 1)) launch_apply_t<T, double>(st, L, E, dx, in, b, out, mode);
   else launch_apply_t<T, T>(st, L, E, dx, in, b, out, mode);
 }
+int64_t apply_dot_partials(int64_t nb) { return nb > RED_MAX_WG ? nb : RED_MAX_WG; }
+template <class T>
+void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, T* out, double* bpart,
+                      double* part, double* dot) {
+  // 256^3 shell, 18 iterations: mixed 42.8 vs 43.1 ms, FP64 60.0 vs 60.5 ms
+  // against the apply followed by a separate dot
+  const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
+  if (sizeof(T) == 4 && ((acc64_modes() >> 2) & 1))
+    k_apply_lds<T, double, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
+  else
+    k_apply_lds<T, T, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
+  if (L.nb <= 65536) {
+    k_sum_wide<<<1, 1024, 0, st>>>(bpart, L.nb, dot);
+  } else {
+    const unsigned g = red_grid(L.nb);
+    k_sum_part<<<g, RED_T, 0, st>>>(bpart, L.nb, part);
+    k_final<0><<<1, RED_T, 0, st>>>(part, (int)g, dot);
+  }
+}
 static ColourOrder colour_order(bool level0, bool forward) {
   ColourOrder o{0};
   for (int i = 0; i < 8; ++i) o.v |= (unsigned)sweep_colour(level0, forward, i) << (4 * i);
@@ -2386,12 +2448,16 @@ static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, cons
   int G = std::min(groups, BC_MAX_WG);
   if (const char* e = std::getenv("SPG_COOP_WG"))
     if (std::atoi(e) > 0) G = std::min(G, std::atoi(e));
-  static const unsigned long long limit = [] {
+  static const unsigned long long wait_50ms = [] {
     int dev = 0, khz = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    return (unsigned long long)khz * 50;  // 50 ms per barrier wait
+    return (unsigned long long)khz * 50;
   }();
+  // SPG_COOP_LIMIT_TICKS shortens the wait (tests force the time-out path)
+  unsigned long long limit = wait_50ms;
+  if (const char* e = std::getenv("SPG_COOP_LIMIT_TICKS"))
+    if (std::atoll(e) > 0) limit = (unsigned long long)std::atoll(e);
   T* ub = reinterpret_cast<T*>(L.b0ub);
   k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt, ub, L.coop);
   BndOff o;
@@ -2672,6 +2738,8 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
 
 #define SPG_INST(T)                                                                                              \
   template void launch_apply<T>(hipStream_t, const LevelDev&, const T*, T, const T*, const T*, T*, int);         \
+  template void launch_apply_dot<T>(hipStream_t, const LevelDev&, const T*, T, const T*, T*, double*, double*,   \
+                                    double*);                                                                   \
   template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int);                     \
   template void launch_gs0_boundary<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);      \
   template void launch_st_residual<T>(hipStream_t, const LevelDev&, const T*, const T*, const T*, T*);           \

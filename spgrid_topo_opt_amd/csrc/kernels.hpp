@@ -87,6 +87,12 @@ void upload_k0(const double* K0, hipStream_t st);
 // level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode);
+// out = P K in (mode 2) and *dot = in . out (deterministic); `bpart` holds
+// apply_dot_partials(nb) doubles, `part` reduce_partials(n) doubles
+int64_t apply_dot_partials(int64_t nb);
+template <class T>
+void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, T* out, double* bpart,
+                      double* part, double* dot);
 // level-0 GS: one colour over all blocks; all boundary sweeps of one phase
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour);

@@ -108,7 +108,12 @@ void Solver::check_coop() {
   unsigned w[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(w, lv_[0].coop.p, sizeof(w), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  if (w[1]) throw SpgError(SPG_E_HIP, "level-0 boundary phase: grid barrier wait timed out");
+  if (w[1]) {
+    // reported once: the next phase starts from a clear word
+    HIPCHK(hipMemsetAsync(static_cast<unsigned*>(lv_[0].coop.p) + 1, 0, sizeof(unsigned), st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    throw SpgError(SPG_E_HIP, "level-0 boundary phase: grid barrier wait timed out");
+  }
 }
 
 void Solver::upload_layout() {
@@ -198,6 +203,7 @@ void Solver::upload_layout() {
   for (DBuf* b : {&x64_, &rr64_, &s64_, &q64_, &z64_, &u_, &f_, &res_, &tmp_}) b->alloc(n3 * 8);
   scal_.alloc(sizeof(CgScalars) + reduce_partials(0) * sizeof(double));
   part_.alloc((reduce_partials(0) + 16) * sizeof(double));
+  apart_.alloc(apply_dot_partials(lv_[0].d.nb) * sizeof(double));
   HIPCHK(hipHostMalloc((void**)&hscal_, 256));
 }
 
@@ -608,8 +614,7 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
     launch_dot<T>(st_, z, r, n, part, &sc->rho);
     launch_cg_scalars(st_, sc, 0);
     launch_cg_direction<T>(st_, s, z, sc, n);
-    launch_apply<T>(st_, R0.d, E<T>(), dxT<T>(), s, nullptr, q, 2);
-    launch_dot<T>(st_, s, q, n, part, &sc->sq);
+    launch_apply_dot<T>(st_, R0.d, E<T>(), dxT<T>(), s, q, apart_.as<double>(), part, &sc->sq);
     launch_cg_scalars(st_, sc, 1);
     launch_cg_update<T>(st_, x, r, s, q, sc, n);
     launch_cg_scalars(st_, sc, 2);

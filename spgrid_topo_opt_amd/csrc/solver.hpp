@@ -106,7 +106,8 @@ class Solver {
   void upload_layout();
   double read_scalar(const double* dev);
   void sync();
-  // throws if a cooperative boundary phase reported a barrier time-out
+  // throws (once per event) if a cooperative boundary phase reported a
+  // barrier time-out
   void check_coop();
 
   spg_config cfg_;
@@ -128,6 +129,7 @@ class Solver {
   DBuf x32_, rr32_, s32_, q32_, z32_, x64_, rr64_, s64_, q64_, z64_;
   DBuf u_, f_, res_, tmp_;
   DBuf scal_, part_, pinned_;
+  DBuf apart_;  // per-workgroup partials of the fused operator apply + dot
   double* hscal_ = nullptr;
   double setup_s_ = 0.0;
   double dev_bytes_ = 0.0;

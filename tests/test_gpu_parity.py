@@ -299,6 +299,26 @@ def test_cooperative_boundary_phase_matches_list_path(wide_face, wg, monkeypatch
     assert "not launched" not in capfd.readouterr().err
 
 
+def test_cooperative_barrier_time_out_is_reported(wide_face, monkeypatch):
+    """a barrier wait cut to one wall-clock tick (set from the host) makes
+    the workgroups leave the kernel early; the call reports the time-out as
+    an error, once, and the next phase runs normally"""
+    s = wide_face
+    gx, gf = s.level_nodes(0)
+    rng = np.random.default_rng(22)
+    u = np.zeros((len(gx), 3))
+    b = rng.standard_normal((len(gx), 3))
+    monkeypatch.setenv("SPG_BOUNDARY_PATH", "5")
+    monkeypatch.setenv("SPG_COOP_LIMIT_TICKS", "1")
+    with pytest.raises(_lib.SpgError, match="timed out"):
+        s.kernel(_lib.OP_GS_BOUNDARY, 0, u, b, precision=0, forward=True, sweeps=3)
+    monkeypatch.delenv("SPG_COOP_LIMIT_TICKS")
+    got = s.kernel(_lib.OP_GS_BOUNDARY, 0, u, b, precision=1, forward=True, sweeps=3)
+    monkeypatch.setenv("SPG_BOUNDARY_PATH", "2")
+    want = s.kernel(_lib.OP_GS_BOUNDARY, 0, u, b, precision=1, forward=True, sweeps=3)
+    assert rel(got, want) < 1e-12
+
+
 @pytest.mark.parametrize("levels", [0, 1, 3, 4])
 def test_fp64_solve_matches_oracle(levels):
     P = Pair(W.random_blocky(seed=21, n=(16, 12, 12), mg_levels=levels))
