@@ -1552,7 +1552,7 @@ __global__ void __launch_bounds__(B0_THREADS) k_bnd0_sweeps(int n, BndOff L, Col
   }
 }
 
-// Level-0 boundary phase over several CUs: a cooperative launch of G
+// Level-0 boundary phase over several CUs: one launch of G <= 32
 // workgroups (all resident at once), each of 8 waves; wave I computes cell
 // corner I's row for 64 nodes of a colour, the corner rows meet in LDS and
 // wave 0 updates the nodes.  The boundary iterate lives in `ub` ([3][n+1],
@@ -2430,8 +2430,8 @@ static bool launch_bnd0_phase(hipStream_t st, const LevelDev& L, T dx, T* u, con
                                                L.b0node, bD, bt, dx, u);
   return true;
 }
-// level-0 boundary phase as one cooperative launch over several CUs; false
-// if the tables are absent or the launch is refused (the caller then runs
+// level-0 boundary phase as one launch over several CUs; false if the
+// tables are absent or a cooperative launch is refused (the caller then runs
 // another path; `loud` reports a refused launch on stderr).  SPG_COOP_WG caps the workgroup count (tests use it to make
 // a workgroup take several node groups per colour step).
 template <class T>
@@ -2470,6 +2470,18 @@ static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, cons
   unsigned long long lim = limit;
   int nn = n, sw = sweeps;
   void* args[] = {&nn, &o, &ord, &sw, &list, &flags, &bE, &cn, &bD, &bt, &dx, &ub, &u, &coop, &lim};
+  // An ordinary launch by default: the solver's stream runs one kernel at a
+  // time, so the G <= 32 small workgroups all start at once on the empty
+  // device (256 CUs).  The cooperative launch API guarantees that even
+  // while other streams fill the device, but costs about 40 us per phase
+  // (256^3 shell: phase 96 vs 115 us, mixed solve 40.2 vs 42.9 ms);
+  // SPG_COOP_LAUNCH=1 selects it.  Either way a barrier wait ends after
+  // `limit` and is reported.
+  const char* cl = std::getenv("SPG_COOP_LAUNCH");
+  if (!(cl && std::atoi(cl) == 1)) {
+    k_bnd0_coop<T><<<G, BC_THREADS, 0, st>>>(nn, o, ord, sw, list, flags, bE, cn, bD, bt, dx, ub, u, coop, lim);
+    return true;
+  }
   const hipError_t e = hipLaunchCooperativeKernel((const void*)k_bnd0_coop<T>, dim3(G), dim3(BC_THREADS), args, 0, st);
   if (e == hipSuccess) return true;
   (void)hipGetLastError();

@@ -112,7 +112,9 @@ void Solver::check_coop() {
     // reported once: the next phase starts from a clear word
     HIPCHK(hipMemsetAsync(static_cast<unsigned*>(lv_[0].coop.p) + 1, 0, sizeof(unsigned), st_));
     HIPCHK(hipStreamSynchronize(st_));
-    throw SpgError(SPG_E_HIP, "level-0 boundary phase: grid barrier wait timed out");
+    throw SpgError(SPG_E_HIP,
+                   "level-0 boundary phase: grid barrier wait timed out (workgroups not resident together; "
+                   "SPG_COOP_LAUNCH=1 uses the cooperative launch API, SPG_BOUNDARY_PATH=2 per-colour launches)");
   }
 }
 

@@ -273,15 +273,18 @@ def wide_face():
     return _lib.Solver(W.box(64, 32, 32, seed=3, mg_levels=3))
 
 
+@pytest.mark.parametrize("launch", ["0", "1"])
 @pytest.mark.parametrize("wg", ["", "1", "2", "4"])
-def test_cooperative_boundary_phase_matches_list_path(wide_face, wg, monkeypatch, capfd):
-    """the cooperative level-0 phase (grid barriers between colour steps,
-    SPG_COOP_WG workgroups, each taking several node groups when capped)
+def test_cooperative_boundary_phase_matches_list_path(wide_face, wg, launch, monkeypatch, capfd):
+    """the single-launch level-0 phase (grid barriers between colour steps,
+    SPG_COOP_WG workgroups, each taking several node groups when capped;
+    ordinary launch or, SPG_COOP_LAUNCH=1, the cooperative launch API)
     against the per-colour list launches, whole phases in both colour
     orders; FP64 to 1e-12, FP32 to 1e-5 relative (only the summation order
     differs)"""
     s = wide_face
     monkeypatch.setenv("SPG_COOP_WG", wg)
+    monkeypatch.setenv("SPG_COOP_LAUNCH", launch)
     gx, gf = s.level_nodes(0)
     n = len(gx)
     rng = np.random.default_rng(21)
