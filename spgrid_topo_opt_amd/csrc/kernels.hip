@@ -1011,12 +1011,13 @@ __global__ void k_bnd_coef_st(int n, const int* __restrict__ list, const double*
 // b~_i = b_i - sum over spokes to nodes outside the boundary set (bt is
 // padded to 4 entries per node).  With `ub` given it also copies the
 // boundary iterate to ub ([3][n+1], zero sentinel at n) and clears the grid
-// barrier counter `ctr` of the cooperative phase.
+// barrier counter `ctr` of the single-launch phase; with `xmap` given it
+// writes the dense map's input [u_B (3n), b~ (3n)].
 template <class T>
 __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ list, const int* __restrict__ bslot,
                                                  const T* __restrict__ A, const T* __restrict__ u,
                                                  const T* __restrict__ b, T* __restrict__ bt, T* __restrict__ ub = nullptr,
-                                                 unsigned* __restrict__ ctr = nullptr) {
+                                                 unsigned* __restrict__ ctr = nullptr, T* __restrict__ xmap = nullptr) {
   const int i = blockIdx.x * 32 + (threadIdx.x >> 3), g = threadIdx.x & 7;
   if (ub && blockIdx.x == 0 && threadIdx.x < 3) ub[threadIdx.x * (n + 1) + n] = T(0);
   if (ctr && blockIdx.x == 0 && threadIdx.x == 0) *ctr = 0u;
@@ -1051,6 +1052,13 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
     if (ub)
 #pragma unroll
       for (int c = 0; c < 3; ++c) ub[c * (n + 1) + i] = u[vidx(blk, c, l)];
+    if (xmap) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xmap[3 * i + c] = u[vidx(blk, c, l)];
+      xmap[3 * n + 3 * i + 0] = b[vidx(blk, 0, l)] - a0;
+      xmap[3 * n + 3 * i + 1] = b[vidx(blk, 1, l)] - a1;
+      xmap[3 * n + 3 * i + 2] = b[vidx(blk, 2, l)] - a2;
+    }
   }
 }
 
@@ -1326,15 +1334,6 @@ __global__ void __launch_bounds__(64 * BM_WAVES) k_bmap_build(int n, BndOff L, C
 // streams it with 16-byte loads, four in flight per lane, and reduces in a
 // fixed order.
 template <class T>
-__global__ void k_bmap_gather(int n, const int* __restrict__ list, const T* __restrict__ u,
-                              const T* __restrict__ bt, T* __restrict__ x) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 3 * n) return;
-  const int i = t / 3, c = t - 3 * (t / 3), node = list[i];
-  x[t] = u[vidx(node >> 6, c, node & 63)];
-  x[3 * n + t] = bt[i * 4 + c];
-}
-template <class T>
 __device__ __forceinline__ T dot4(const typename Vec4<T>::type& a, const typename Vec4<T>::type& b, T acc);
 template <>
 __device__ __forceinline__ float dot4<float>(const float4& a, const float4& b, float acc) {
@@ -1344,9 +1343,12 @@ template <>
 __device__ __forceinline__ double dot4<double>(const double2& a, const double2& b, double acc) {
   return fma(a.y, b.y, fma(a.x, b.x, acc));
 }
+// y = M x, one wavefront per row; with `list` given, row 3i+c goes straight
+// to u at boundary node list[i], component c
 template <class T>
 __global__ void __launch_bounds__(256) k_bmap_gemv(int rows, int Kp, const T* __restrict__ M, const T* __restrict__ x,
-                                                   T* __restrict__ y) {
+                                                   T* __restrict__ y, const int* __restrict__ list = nullptr,
+                                                   T* __restrict__ u = nullptr) {
   using V = typename Vec4<T>::type;
   constexpr int PER = 16 / sizeof(T);
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1367,14 +1369,14 @@ __global__ void __launch_bounds__(256) k_bmap_gemv(int rows, int Kp, const T* __
   T acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) y[r] = acc;
-}
-template <class T>
-__global__ void k_bmap_scatter(int n, const int* __restrict__ list, const T* __restrict__ y, T* __restrict__ u) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 3 * n) return;
-  const int i = t / 3, c = t - 3 * (t / 3), node = list[i];
-  u[vidx(node >> 6, c, node & 63)] = y[t];
+  if (lane == 0) {
+    if (list) {
+      const int i = r / 3, c = r - 3 * i, node = list[i];
+      u[vidx(node >> 6, c, node & 63)] = acc;
+    } else {
+      y[r] = acc;
+    }
+  }
 }
 // ------------------------------------- level-0 matrix-free boundary phase
 // Level 0 needs no coefficient rows: boundary node i carries the moduli of
@@ -1736,9 +1738,12 @@ struct BndThreads {
 // null-space projections (Prolongation.h:57-99, Multigrid_Hierarchy.h:600-603).
 static constexpr int TR_WAVES = 2;
 template <class T>
+// With `czero` given, the kernel also clears that vector on the coarse
+// level (the coarse iterate of the V-cycle), saving a launch.
 __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const int* __restrict__ rsrc,
                                                                 const uint8_t* __restrict__ cflags,
-                                                                const T* __restrict__ fine, T* __restrict__ coarse) {
+                                                                const T* __restrict__ fine, T* __restrict__ coarse,
+                                                                T* __restrict__ czero) {
   __shared__ T hf[TR_WAVES][3][729];
   __shared__ int ht[TR_WAVES][27];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1746,12 +1751,25 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const i
   const bool live = blk < nbc;
   if (lane < 27) ht[w][lane] = live ? rsrc[blk * 27 + lane] : -1;
   __syncthreads();
-  for (int h = lane; h < 729; h += 64) {
-    const int x = h / 81 - 1, y = (h / 9) % 9 - 1, z = h % 9 - 1;
-    const int fb = ht[w][((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((z >> 2) + 1)];
-    const int q = loc3(x, y, z);
+  {
+    // the 9^3 fine halo in 12 passes; all loads are issued before the LDS writes
+    constexpr int NP = (729 + 63) / 64;
+    T v[NP][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hf[w][c][h] = fb >= 0 ? fine[vidx(fb, c, q)] : T(0);
+    for (int i = 0; i < NP; ++i) {
+      const int h = lane + 64 * i;
+      const int hh = h < 729 ? h : 0;
+      const int x = hh / 81 - 1, y = (hh / 9) % 9 - 1, z = hh % 9 - 1;
+      const int fb = h < 729 ? ht[w][((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((z >> 2) + 1)] : -1;
+      const int q = loc3(x, y, z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = fb >= 0 ? fine[vidx(fb, c, q)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+      if (lane + 64 * i < 729)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hf[w][c][lane + 64 * i] = v[i][c];
   }
   __syncthreads();
   if (!live) return;
@@ -1773,6 +1791,9 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const i
   coarse[vidx(blk, 0, lane)] = (fl & 2) ? T(0) : a0;
   coarse[vidx(blk, 1, lane)] = (fl & 4) ? T(0) : a1;
   coarse[vidx(blk, 2, lane)] = (fl & 8) ? T(0) : a2;
+  if (czero)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) czero[vidx(blk, c, lane)] = T(0);
 }
 
 template <class T>
@@ -1980,6 +2001,26 @@ __global__ void k_cg_scalars(CgScalars* sc, int stage) {
   }
 }
 
+// v = P P_null v and partial sums of v . w, node by node (grid-stride)
+template <class T>
+__global__ void __launch_bounds__(RED_T) k_project_dot_part(int nb, const uint8_t* __restrict__ flags, T* v,
+                                                            const T* __restrict__ w, double* part) {
+  double s = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < (int64_t)nb * 64;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int blk = (int)(g >> 6), l = (int)(g & 63);
+    const uint8_t fl = flags[g];
+    const bool dead = !(fl & 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int k = vidx(blk, c, l);
+      if (dead || (fl & (2 << c))) v[k] = T(0);
+      else s += (double)v[k] * (double)w[k];
+    }
+  }
+  s = block_reduce<0>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
 template <class T>
 __global__ void k_project(int nb, const uint8_t* flags, T* v, int nullspace) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2600,10 +2641,11 @@ void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, co
     const int Kp = (int)bmap_stride(n);
     T* x = BmapPtr<T>::x(L);  // [Kp] input (zero past 6n), then [3n] output behind it
     T* bt = BndPtr<T>::bt(L);
-    k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, BndPtr<T>::A(L), u, b, bt);
-    k_bmap_gather<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, u, bt, x);
-    k_bmap_gemv<T><<<(3 * n + 3) / 4, 256, 0, st>>>(3 * n, Kp, M, x, x + Kp);
-    k_bmap_scatter<T><<<nblk(3 * n), TPB, 0, st>>>(n, L.bnd, x + Kp, u);
+    // two launches: b~ and the map input, then the map with the result
+    // written to u (the separate gather / scatter kernels cost two launch
+    // gaps per phase)
+    k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, BndPtr<T>::A(L), u, b, bt, nullptr, nullptr, x);
+    k_bmap_gemv<T><<<(3 * n + 3) / 4, 256, 0, st>>>(3 * n, Kp, M, x, x + Kp, L.bnd, u);
     return;
   }
   if (path == 0 && launch_bnd_phase<T>(st, L, u, b, sweeps, false, forward)) return;
@@ -2619,8 +2661,9 @@ void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, co
     }
 }
 template <class T>
-void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse) {
-  k_restrict_lds<T><<<(C.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(C.nb, C.rsrc, C.flags, fine, coarse);
+void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse, T* coarse_zero) {
+  k_restrict_lds<T><<<(C.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(C.nb, C.rsrc, C.flags, fine, coarse,
+                                                                              coarse_zero);
 }
 template <class T>
 void launch_prolongate(hipStream_t st, const LevelDev& F, const T* coarse, T* fine) {
@@ -2675,6 +2718,12 @@ void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, const 
   k_final<1><<<1, RED_T, 0, st>>>(part, (int)g, &const_cast<CgScalars*>(sc)->nrm);
 }
 void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage) { k_cg_scalars<<<1, 64, 0, st>>>(sc, stage); }
+template <class T>
+void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out) {
+  const unsigned g = red_grid((int64_t)L.nb * 64);
+  k_project_dot_part<T><<<g, RED_T, 0, st>>>(L.nb, L.flags, v, w, partial);
+  k_final<0><<<1, RED_T, 0, st>>>(partial, (int)g, out);
+}
 template <class T>
 void launch_project(hipStream_t st, const LevelDev& L, T* v, bool nullspace) {
   k_project<T><<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.flags, v, nullspace ? 1 : 0);
@@ -2757,7 +2806,7 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_st_residual<T>(hipStream_t, const LevelDev&, const T*, const T*, const T*, T*);           \
   template void launch_st_gs<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int);                      \
   template void launch_st_boundary<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int, bool);          \
-  template void launch_restrict<T>(hipStream_t, const LevelDev&, const T*, T*);                                  \
+  template void launch_restrict<T>(hipStream_t, const LevelDev&, const T*, T*, T*);                              \
   template void launch_prolongate<T>(hipStream_t, const LevelDev&, const T*, T*);                                \
   template void launch_zero<T>(hipStream_t, T*, int64_t);                                                        \
   template void launch_copy<T>(hipStream_t, const T*, T*, int64_t);                                              \
@@ -2767,6 +2816,7 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_cg_direction<T>(hipStream_t, T*, const T*, const CgScalars*, int64_t);                    \
   template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, const CgScalars*, int64_t);         \
   template void launch_project<T>(hipStream_t, const LevelDev&, T*, bool);                                       \
+  template void launch_project_dot<T>(hipStream_t, const LevelDev&, T*, const T*, double*, double*);             \
   template void launch_bottom_gather<T>(hipStream_t, const LevelDev&, const int*, const T*, T*);                 \
   template void launch_bottom_scatter<T>(hipStream_t, const LevelDev&, const int*, const T*, const T*, T*);      \
   template void launch_pack_sym_tiles<T>(hipStream_t, const double*, int64_t, int, T*);                          \

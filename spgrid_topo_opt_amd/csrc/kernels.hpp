@@ -107,8 +107,9 @@ void launch_st_gs(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T*
 template <class T>
 void launch_st_boundary(hipStream_t st, const LevelDev& L, const T* ST, T* u, const T* b, int sweeps, bool forward);
 // transfers
+// coarse = R fine; `coarse_zero` (optional) is cleared in the same launch
 template <class T>
-void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse);
+void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse, T* coarse_zero = nullptr);
 template <class T>
 void launch_prolongate(hipStream_t st, const LevelDev& F, const T* coarse, T* fine);
 // vectors
@@ -136,6 +137,9 @@ void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage);  // 0: after r
 // projections
 template <class T>
 void launch_project(hipStream_t st, const LevelDev& L, T* v, bool nullspace);
+// v = P P_null v, then *out = v . w (deterministic two-stage sum)
+template <class T>
+void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out);
 // Galerkin assembly (FP64)
 void launch_galerkin_gather(hipStream_t st, int64_t ncell, int depth, const int* cell_fine_blocks,
                             const int* cell_gpu_index, const double* E0, double* G);

@@ -551,12 +551,12 @@ void Solver::vcycle(const T* b0, T* z) {
     const T* b = l == 0 ? b0 : lv_[l].b<T>();
     const bool sm = (l == 0 || l >= ex);
     if (sm) smooth<T>(l, u, b, true);
-    launch_zero<T>(st_, lv_[l + 1].u<T>(), lv_[l + 1].n3);
+    // the restriction also clears the coarse iterate
     if (sm) {
       level_residual<T>(l, u, b, lv_[l].r<T>());
-      launch_restrict<T>(st_, lv_[l + 1].d, lv_[l].r<T>(), lv_[l + 1].b<T>());
+      launch_restrict<T>(st_, lv_[l + 1].d, lv_[l].r<T>(), lv_[l + 1].b<T>(), lv_[l + 1].u<T>());
     } else {
-      launch_restrict<T>(st_, lv_[l + 1].d, b, lv_[l + 1].b<T>());
+      launch_restrict<T>(st_, lv_[l + 1].d, b, lv_[l + 1].b<T>(), lv_[l + 1].u<T>());
     }
   }
   bottom<T>(lv_[L - 1].b<T>(), lv_[L - 1].u<T>());
@@ -612,8 +612,7 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
     }
     if (it == maxit) break;
     vcycle<T>(r, z);
-    launch_project<T>(st_, R0.d, z, true);
-    launch_dot<T>(st_, z, r, n, part, &sc->rho);
+    launch_project_dot<T>(st_, R0.d, z, r, part, &sc->rho);
     launch_cg_scalars(st_, sc, 0);
     launch_cg_direction<T>(st_, s, z, sc, n);
     launch_apply_dot<T>(st_, R0.d, E<T>(), dxT<T>(), s, q, apart_.as<double>(), part, &sc->sq);
