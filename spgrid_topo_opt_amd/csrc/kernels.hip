@@ -1806,6 +1806,10 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_prolong_lds(int nbf, const in
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * TR_WAVES + w;
   const bool live = blk < nbf;
   const int b0 = live ? blk : 0;
+  // the fine block's own values do not depend on the staged coarse halo:
+  // their loads go out first
+  const uint8_t fl = fflags[b0 * 64 + lane];
+  T a0 = fine[vidx(b0, 0, lane)], a1 = fine[vidx(b0, 1, lane)], a2 = fine[vidx(b0, 2, lane)];
   const int phx = ((fbase[3 * b0] + 4) >> 1) & 3, phy = ((fbase[3 * b0 + 1] + 4) >> 1) & 3,
             phz = ((fbase[3 * b0 + 2] + 8) >> 1) & 3;
   if (lane < 27) {
@@ -1818,8 +1822,6 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_prolong_lds(int nbf, const in
   __syncthreads();
   if (!live) return;
   const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
-  const uint8_t fl = fflags[blk * 64 + lane];
-  T a0 = fine[vidx(blk, 0, lane)], a1 = fine[vidx(blk, 1, lane)], a2 = fine[vidx(blk, 2, lane)];
 #pragma unroll
   for (int oi = 0; oi < 2; ++oi)
 #pragma unroll
