@@ -1927,12 +1927,31 @@ __global__ void __launch_bounds__(RED_T) k_sumsq_part(const double* a, int64_t n
   s = block_reduce<0>(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
+// PCG scalar updates, run by the thread that has just written the
+// reduction they depend on (stage 0 after rho, 1 after s.q, 2 after ||r||)
+__device__ inline void cg_stage(CgScalars* sc, int stage) {
+  if (stage == 0) {
+    sc->beta = sc->rho / sc->rho_old;
+  } else if (stage == 1) {
+    sc->singular = !(sc->sq > 0.0);
+    sc->alpha = sc->sq != 0.0 ? sc->rho / sc->sq : (double)FLT_MAX;
+  } else if (stage == 2) {
+    sc->rho_old = sc->rho;
+  } else if (stage == 3) {
+    sc->rho_old = (double)FLT_MAX;
+    sc->singular = 0;
+  }
+}
 template <int OP>
-__global__ void __launch_bounds__(RED_T) k_final(const double* part, int np, double* out) {
+__global__ void __launch_bounds__(RED_T) k_final(const double* part, int np, double* out, CgScalars* sc = nullptr,
+                                                 int stage = -1) {
   double s = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) s = OP == 0 ? s + part[i] : fmax(s, part[i]);
   s = block_reduce<OP>(s);
-  if (threadIdx.x == 0) *out = s;
+  if (threadIdx.x == 0) {
+    *out = s;
+    if (sc) cg_stage(sc, stage);
+  }
 }
 
 __global__ void __launch_bounds__(RED_T) k_sum_part(const double* a, int64_t n, double* part) {
@@ -1943,7 +1962,8 @@ __global__ void __launch_bounds__(RED_T) k_sum_part(const double* a, int64_t n, 
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 // sum of np partials in a fixed order by one 1024-thread workgroup
-__global__ void __launch_bounds__(1024) k_sum_wide(const double* __restrict__ part, int np, double* out) {
+__global__ void __launch_bounds__(1024) k_sum_wide(const double* __restrict__ part, int np, double* out,
+                                                   CgScalars* sc = nullptr, int stage = -1) {
   __shared__ double sh[16];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   int i = threadIdx.x;
@@ -1963,6 +1983,7 @@ __global__ void __launch_bounds__(1024) k_sum_wide(const double* __restrict__ pa
     double t = sh[0];
     for (int q = 1; q < 16; ++q) t += sh[q];
     *out = t;
+    if (sc) cg_stage(sc, stage);
   }
 }
 
@@ -1989,18 +2010,7 @@ __global__ void __launch_bounds__(RED_T) k_cg_update(T* x, T* r, const T* s, con
   if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 __global__ void k_cg_scalars(CgScalars* sc, int stage) {
-  if (threadIdx.x != 0) return;
-  if (stage == 0) {
-    sc->beta = sc->rho / sc->rho_old;
-  } else if (stage == 1) {
-    sc->singular = !(sc->sq > 0.0);
-    sc->alpha = sc->sq != 0.0 ? sc->rho / sc->sq : (double)FLT_MAX;
-  } else if (stage == 2) {
-    sc->rho_old = sc->rho;
-  } else if (stage == 3) {
-    sc->rho_old = (double)FLT_MAX;
-    sc->singular = 0;
-  }
+  if (threadIdx.x == 0) cg_stage(sc, stage);
 }
 
 // v = P P_null v and partial sums of v . w, node by node (grid-stride)
@@ -2362,7 +2372,7 @@ void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* 
 int64_t apply_dot_partials(int64_t nb) { return nb > RED_MAX_WG ? nb : RED_MAX_WG; }
 template <class T>
 void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, T* out, double* bpart,
-                      double* part, double* dot) {
+                      double* part, double* dot, CgScalars* sc, int stage) {
   // 256^3 shell, 18 iterations: mixed 42.8 vs 43.1 ms, FP64 60.0 vs 60.5 ms
   // against the apply followed by a separate dot
   const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
@@ -2370,12 +2380,16 @@ void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const
     k_apply_lds<T, double, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
   else
     k_apply_lds<T, T, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
-  if (L.nb <= 65536) {
-    k_sum_wide<<<1, 1024, 0, st>>>(bpart, L.nb, dot);
+  // one workgroup sums up to 65536 partials; beyond that two stages
+  // (SPG_SUM_WIDE_MAX lowers the limit so that tests reach both forms)
+  int wide_max = 65536;
+  if (const char* e = std::getenv("SPG_SUM_WIDE_MAX")) wide_max = std::atoi(e);
+  if (L.nb <= wide_max) {
+    k_sum_wide<<<1, 1024, 0, st>>>(bpart, L.nb, dot, sc, stage);
   } else {
     const unsigned g = red_grid(L.nb);
     k_sum_part<<<g, RED_T, 0, st>>>(bpart, L.nb, part);
-    k_final<0><<<1, RED_T, 0, st>>>(part, (int)g, dot);
+    k_final<0><<<1, RED_T, 0, st>>>(part, (int)g, dot, sc, stage);
   }
 }
 static ColourOrder colour_order(bool level0, bool forward) {
@@ -2712,19 +2726,20 @@ void launch_cg_direction(hipStream_t st, T* s, const T* z, const CgScalars* sc, 
   k_cg_direction<T><<<nblk(n), TPB, 0, st>>>(s, z, sc, n);
 }
 template <class T>
-void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, const CgScalars* sc, int64_t n) {
+void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage) {
   // caller passes the partial buffer right after the scalars block
-  double* part = reinterpret_cast<double*>(const_cast<CgScalars*>(sc) + 1);
+  double* part = reinterpret_cast<double*>(sc + 1);
   const unsigned g = red_grid(n);
   k_cg_update<T><<<g, RED_T, 0, st>>>(x, r, s, q, sc, n, part);
-  k_final<1><<<1, RED_T, 0, st>>>(part, (int)g, &const_cast<CgScalars*>(sc)->nrm);
+  k_final<1><<<1, RED_T, 0, st>>>(part, (int)g, &sc->nrm, stage >= 0 ? sc : nullptr, stage);
 }
 void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage) { k_cg_scalars<<<1, 64, 0, st>>>(sc, stage); }
 template <class T>
-void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out) {
+void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out,
+                        CgScalars* sc, int stage) {
   const unsigned g = red_grid((int64_t)L.nb * 64);
   k_project_dot_part<T><<<g, RED_T, 0, st>>>(L.nb, L.flags, v, w, partial);
-  k_final<0><<<1, RED_T, 0, st>>>(partial, (int)g, out);
+  k_final<0><<<1, RED_T, 0, st>>>(partial, (int)g, out, sc, stage);
 }
 template <class T>
 void launch_project(hipStream_t st, const LevelDev& L, T* v, bool nullspace) {
@@ -2802,7 +2817,7 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
 #define SPG_INST(T)                                                                                              \
   template void launch_apply<T>(hipStream_t, const LevelDev&, const T*, T, const T*, const T*, T*, int);         \
   template void launch_apply_dot<T>(hipStream_t, const LevelDev&, const T*, T, const T*, T*, double*, double*,   \
-                                    double*);                                                                   \
+                                    double*, CgScalars*, int);                                                  \
   template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int);                     \
   template void launch_gs0_boundary<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);      \
   template void launch_st_residual<T>(hipStream_t, const LevelDev&, const T*, const T*, const T*, T*);           \
@@ -2816,9 +2831,10 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_dot<T>(hipStream_t, const T*, const T*, int64_t, double*, double*);                       \
   template void launch_linf<T>(hipStream_t, const T*, int64_t, double*, double*);                                \
   template void launch_cg_direction<T>(hipStream_t, T*, const T*, const CgScalars*, int64_t);                    \
-  template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, const CgScalars*, int64_t);         \
+  template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, CgScalars*, int64_t, int);         \
   template void launch_project<T>(hipStream_t, const LevelDev&, T*, bool);                                       \
-  template void launch_project_dot<T>(hipStream_t, const LevelDev&, T*, const T*, double*, double*);             \
+  template void launch_project_dot<T>(hipStream_t, const LevelDev&, T*, const T*, double*, double*, CgScalars*,  \
+                                      int);                                                                     \
   template void launch_bottom_gather<T>(hipStream_t, const LevelDev&, const int*, const T*, T*);                 \
   template void launch_bottom_scatter<T>(hipStream_t, const LevelDev&, const int*, const T*, const T*, T*);      \
   template void launch_pack_sym_tiles<T>(hipStream_t, const double*, int64_t, int, T*);                          \

@@ -92,7 +92,7 @@ void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* 
 int64_t apply_dot_partials(int64_t nb);
 template <class T>
 void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, T* out, double* bpart,
-                      double* part, double* dot);
+                      double* part, double* dot, CgScalars* sc = nullptr, int stage = -1);
 // level-0 GS: one colour over all blocks; all boundary sweeps of one phase
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour);
@@ -132,14 +132,16 @@ void launch_sumsq(hipStream_t st, const double* a, int64_t n, double* partial, d
 template <class T>
 void launch_cg_direction(hipStream_t st, T* s, const T* z, const CgScalars* sc, int64_t n);  // s = beta s + z
 template <class T>
-void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, const CgScalars* sc, int64_t n);
+void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage = -1);
 void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage);  // 0: after rho, 1: after sq, 2: end
+// (the reductions above take `sc, stage` to run a stage in their last kernel)
 // projections
 template <class T>
 void launch_project(hipStream_t st, const LevelDev& L, T* v, bool nullspace);
 // v = P P_null v, then *out = v . w (deterministic two-stage sum)
 template <class T>
-void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out);
+void launch_project_dot(hipStream_t st, const LevelDev& L, T* v, const T* w, double* partial, double* out,
+                        CgScalars* sc = nullptr, int stage = -1);
 // Galerkin assembly (FP64)
 void launch_galerkin_gather(hipStream_t st, int64_t ncell, int depth, const int* cell_fine_blocks,
                             const int* cell_gpu_index, const double* E0, double* G);

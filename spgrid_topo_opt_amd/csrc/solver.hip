@@ -612,13 +612,12 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
     }
     if (it == maxit) break;
     vcycle<T>(r, z);
-    launch_project_dot<T>(st_, R0.d, z, r, part, &sc->rho);
-    launch_cg_scalars(st_, sc, 0);
+    // the scalar stages (beta, alpha, rho_old) run in the last kernel of
+    // the reduction they follow
+    launch_project_dot<T>(st_, R0.d, z, r, part, &sc->rho, sc, 0);
     launch_cg_direction<T>(st_, s, z, sc, n);
-    launch_apply_dot<T>(st_, R0.d, E<T>(), dxT<T>(), s, q, apart_.as<double>(), part, &sc->sq);
-    launch_cg_scalars(st_, sc, 1);
-    launch_cg_update<T>(st_, x, r, s, q, sc, n);
-    launch_cg_scalars(st_, sc, 2);
+    launch_apply_dot<T>(st_, R0.d, E<T>(), dxT<T>(), s, q, apart_.as<double>(), part, &sc->sq, sc, 1);
+    launch_cg_update<T>(st_, x, r, s, q, sc, n, 2);
   }
   *iters = it;
   return rc;

@@ -373,6 +373,19 @@ def test_mixed_solve_matches_fp64_oracle(box3):
     assert abs(fu_o / P.p.dx - c_o) / c_o < 1e-9
 
 
+@pytest.mark.parametrize("wide_max", ["0", "65536"])
+def test_operator_dot_sum_forms_agree(box3, wide_max, monkeypatch):
+    """the s.q partials of the PCG operator are summed by one workgroup up to
+    SPG_SUM_WIDE_MAX blocks and in two stages beyond; both forms give the
+    oracle's solution (FP64 solve to 1e-10, displacement error 1e-8)"""
+    P = box3
+    monkeypatch.setenv("SPG_SUM_WIDE_MAX", wide_max)
+    u_o, _, _ = P.o.solve(tolerance=1e-12, max_iterations=500)
+    out, ig = P.g.solve(tolerance=1e-10, precision=1, max_passes=1, max_iterations=500)
+    assert ig["converged"] == 1
+    assert _solution_error(P, out, u_o) < 1e-8
+
+
 def test_cfg1_reference_pass_and_mixed():
     P = Pair(W.config("cfg1"))
     u_o, _, io = P.o.solve(tolerance=1e-10, max_iterations=500)
