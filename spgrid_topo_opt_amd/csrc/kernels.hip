@@ -621,39 +621,52 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
   }
   if (tid < 125) htab[tid] = cHaloC[colour * 125 + tid];
   __syncthreads();
+  // lane roles after staging: block bl, node k of the colour, corner pair q
+  const int bl = tid >> 5, k = (tid >> 2) & 7, q = tid & 3;
+  const int blk = blk0 + bl;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;  // halo coords
+  const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+  T ee[2], bv[3] = {T(0), T(0), T(0)};
+  uint8_t fl = 0;
   {
-    // 32 threads per block stage its 5^3 halo (4 passes, table in LDS); all
-    // loads are issued before one wait
-    const int sb = tid >> 5, r = tid & 31;
+    // 32 threads per block stage its 5^3 halo (4 passes, table in LDS).  The
+    // halo loads, this lane's two cell moduli and (lane q = 0) the node's
+    // flags and b are all issued before one wait.
+    const int rs = tid & 31;
     T v[4][3];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int hh = r + 32 * i;
+      const int hh = rs + 32 * i;
       const int t = hh < 125 ? htab[hh] : 0;
-      const int bb = hh < 125 ? ht[sb][t >> 8] : -1;
+      const int bb = hh < 125 ? ht[bl][t >> 8] : -1;
 #pragma unroll
       for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, t & 255)] : T(0);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (r + 32 * i < 125)
+    for (int r = 0; r < 2; ++r) {
+      const int I = q | (r << 2);
+      const int ex = px - 1 + nx - ((I >> 2) & 1), ey = py - 1 + ny - ((I >> 1) & 1), ez = pz - 1 + nz - (I & 1);
+      const int eb = ht[bl][ring_spoke(ex, ey, ez)];
+      ee[r] = eb >= 0 ? E[eb * 64 + loc3(ex, ey, ez)] : T(0);
+    }
+    if (q == 0 && blk < nb) {
+      fl = flags[blk * 64 + l];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) hu[sb][c][r + 32 * i] = v[i][c];
+      for (int c = 0; c < 3; ++c) bv[c] = b[vidx(blk, c, l)];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (rs + 32 * i < 125)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hu[bl][c][rs + 32 * i] = v[i][c];
   }
   __syncthreads();
-  const int bl = tid >> 5, k = (tid >> 2) & 7, q = tid & 3;
-  const int blk = blk0 + bl;
-  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;  // halo coords
   const T* H = &hu[bl][0][0];
-  T f0 = 0, f1 = 0, f2 = 0, ee[2];
+  T f0 = 0, f1 = 0, f2 = 0;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int I = q | (r << 2);
     const int cx = nx - ((I >> 2) & 1), cy = ny - ((I >> 1) & 1), cz = nz - (I & 1);  // cell corner (halo)
-    const int ex = px - 1 + cx, ey = py - 1 + cy, ez = pz - 1 + cz;                   // block coords
-    const int eb = ht[bl][ring_spoke(ex, ey, ez)];
-    const T e = eb >= 0 ? E[eb * 64 + loc3(ex, ey, ez)] : T(0);
-    ee[r] = e;
     auto U = [&](int J, T& v0, T& v1, T& v2) {
       const int h = ((cx + ((J >> 2) & 1)) * 5 + (cy + ((J >> 1) & 1))) * 5 + (cz + (J & 1));
       v0 = H[h];
@@ -661,7 +674,7 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
       v2 = H[256 + h];
     };
     T g0, g1, g2;
-    corner_row<T>(I, e, U, g0, g1, g2);
+    corner_row<T>(I, ee[r], U, g0, g1, g2);
     f0 += g0;
     f1 += g1;
     f2 += g2;
@@ -676,9 +689,8 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
           Syz = dx * quad_sum(sy * sz * es);
   if (q != 0 || blk >= nb) return;
   T d[6] = {KT<T>::k(0) * S, KT<T>::k(4) * S, KT<T>::k(8) * S, KT<T>::k(1) * Sxy, KT<T>::k(2) * Sxz, KT<T>::k(5) * Syz};
-  const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
   const int hn = (nx * 5 + ny) * 5 + nz;
-  gs_node_update<T>(flags[blk * 64 + l], d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], b, u);
+  gs_node_update_b<T>(fl, d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], bv[0], bv[1], bv[2], u);
 }
 
 // Level-0 update of list node `s` (blk*64+l) by the 8 lanes of an octet,
