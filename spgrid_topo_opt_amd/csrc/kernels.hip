@@ -870,9 +870,14 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
   if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
   for (int h = lane; h < 125; h += 64) htab[w][h] = cHaloC[colour * 125 + h];
   __syncthreads();
+  const int k = lane >> 3, g = lane & 7;
+  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;
+  const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
+  uint8_t fl = 0;
+  T bv[3] = {T(0), T(0), T(0)};
   {
     // the 5^3 halo of colour (px,py,pz) from the table in LDS; both passes'
-    // loads are issued before one wait
+    // loads, and the updating lane's flags and b, are issued before one wait
     T v[2][3];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -882,6 +887,11 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
 #pragma unroll
       for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? u[vidx(bb, c, t & 255)] : T(0);
     }
+    if (g == 5 && live) {
+      fl = flags[blk * 64 + l];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bv[c] = b[vidx(blk, c, l)];
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
       if (lane + 64 * i < 125)
@@ -889,8 +899,6 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
         for (int c = 0; c < 3; ++c) hu[w][c][lane + 64 * i] = v[i][c];
   }
   __syncthreads();
-  const int k = lane >> 3, g = lane & 7;
-  const int nx = 2 * ((k >> 2) & 1) + 1, ny = 2 * ((k >> 1) & 1) + 1, nz = 2 * (k & 1) + 1;
   const T* H = &hu[w][0][0];
   auto U = [&](int s, T& x, T& y, T& z) {
     const int h = ((nx + s / 9 - 1) * 5 + (ny + (s / 3) % 3 - 1)) * 5 + (nz + s % 3 - 1);
@@ -902,16 +910,14 @@ __global__ void __launch_bounds__(64 * GS_WAVES) k_st_colour(int nb, int colour0
   T f0, f1, f2, c9[9];
   st_octet_row<T>(k, g, S, U, f0, f1, f2, c9);
   if (g == 5 && live) {
-    const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
-    const uint8_t fl = flags[blk * 64 + l];
     if (MODE == 0) {
       const int hn = (nx * 5 + ny) * 5 + nz;
       T d[6] = {c9[0], c9[4], c9[8], c9[1], c9[2], c9[5]};
-      gs_node_update<T>(fl, d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], b, u);
+      gs_node_update_b<T>(fl, d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], bv[0], bv[1], bv[2], u);
     } else {
       const T f[3] = {f0, f1, f2};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) r[vidx(blk, c, l)] = (fl & (2 << c)) ? T(0) : b[vidx(blk, c, l)] - f[c];
+      for (int c = 0; c < 3; ++c) r[vidx(blk, c, l)] = (fl & (2 << c)) ? T(0) : bv[c] - f[c];
     }
   }
 }
