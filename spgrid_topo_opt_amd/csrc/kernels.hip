@@ -221,6 +221,11 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
   if (threadIdx.x < 216) htab6[threadIdx.x] = cHalo6[threadIdx.x];
   if (threadIdx.x < 125) htab5[threadIdx.x] = cHalo5[threadIdx.x];
   __syncthreads();
+  // FP32 residual: the node's flags and b go out with the halo loads (the
+  // FP64 forms keep them at the end, where they cost no registers)
+  constexpr bool kPre = MODE == 1 && sizeof(T) == 4 && sizeof(A) == 4;
+  uint8_t pfl = 0;
+  T pb[3] = {T(0), T(0), T(0)};
   {
     // 6^3 node halo (block coordinates -1..4) and 5^3 cell halo (-1..3)
     // from the tables in LDS; every pass's loads are issued before the LDS
@@ -240,6 +245,11 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
       const int t = h < 125 ? htab5[h] : 0;
       const int bb = h < 125 ? ht[w][t >> 8] : -1;
       e[i] = bb >= 0 ? E[bb * 64 + (t & 255)] : T(0);
+    }
+    if (kPre && live) {
+      pfl = flags[blk * 64 + lane];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pb[c] = b[vidx(blk, c, lane)];
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -290,11 +300,11 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[vidx(blk, c, lane)] = f[c];
   } else {
-    const uint8_t fl = flags[blk * 64 + lane];
+    const uint8_t fl = kPre ? pfl : flags[blk * 64 + lane];
     double d = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T v = MODE >= 2 ? f[c] : b[vidx(blk, c, lane)] - f[c];
+      const T v = MODE >= 2 ? f[c] : (kPre ? pb[c] : b[vidx(blk, c, lane)]) - f[c];
       const T o = (fl & (2 << c)) ? T(0) : v;
       out[vidx(blk, c, lane)] = o;
       if (MODE == 3) d += (double)hu[w][c][(li + 1) * 36 + (lj + 1) * 6 + (lk + 1)] * (double)o;
@@ -1040,6 +1050,15 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
   if (ub && blockIdx.x == 0 && threadIdx.x < 3) ub[threadIdx.x * (n + 1) + n] = T(0);
   if (ctr && blockIdx.x == 0 && threadIdx.x == 0) *ctr = 0u;
   const bool live = i < n;
+  // the node's own b and u go out with the neighbour loads below
+  const int node = live ? list[i] : 0, blk = node >> 6, l = node & 63;
+  T bn[3] = {T(0), T(0), T(0)}, un[3] = {T(0), T(0), T(0)};
+  if (g == 0 && live)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bn[c] = b[vidx(blk, c, l)];
+      un[c] = u[vidx(blk, c, l)];
+    }
   T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -1062,20 +1081,19 @@ __global__ void __launch_bounds__(256) k_bnd_rhs(int n, const int* __restrict__ 
   a1 = octet_sum(a1);
   a2 = octet_sum(a2);
   if (g == 0 && live) {
-    const int node = list[i], blk = node >> 6, l = node & 63;
-    bt[i * 4 + 0] = b[vidx(blk, 0, l)] - a0;
-    bt[i * 4 + 1] = b[vidx(blk, 1, l)] - a1;
-    bt[i * 4 + 2] = b[vidx(blk, 2, l)] - a2;
+    bt[i * 4 + 0] = bn[0] - a0;
+    bt[i * 4 + 1] = bn[1] - a1;
+    bt[i * 4 + 2] = bn[2] - a2;
     bt[i * 4 + 3] = T(0);
     if (ub)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ub[c * (n + 1) + i] = u[vidx(blk, c, l)];
+      for (int c = 0; c < 3; ++c) ub[c * (n + 1) + i] = un[c];
     if (xmap) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xmap[3 * i + c] = u[vidx(blk, c, l)];
-      xmap[3 * n + 3 * i + 0] = b[vidx(blk, 0, l)] - a0;
-      xmap[3 * n + 3 * i + 1] = b[vidx(blk, 1, l)] - a1;
-      xmap[3 * n + 3 * i + 2] = b[vidx(blk, 2, l)] - a2;
+      for (int c = 0; c < 3; ++c) xmap[3 * i + c] = un[c];
+      xmap[3 * n + 3 * i + 0] = bn[0] - a0;
+      xmap[3 * n + 3 * i + 1] = bn[1] - a1;
+      xmap[3 * n + 3 * i + 2] = bn[2] - a2;
     }
   }
 }
