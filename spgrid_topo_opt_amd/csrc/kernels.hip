@@ -2032,8 +2032,10 @@ __global__ void k_cg_direction(T* s, const T* z, const CgScalars* sc, int64_t n)
   s[i] = beta * s[i] + z[i];
 }
 template <class T>
+// x += alpha s, r -= alpha q, partial max |r|; with `zc` also clears zc
+// (the next V-cycle's iterate)
 __global__ void __launch_bounds__(RED_T) k_cg_update(T* x, T* r, const T* s, const T* q, const CgScalars* sc, int64_t n,
-                                                      double* part) {
+                                                      double* part, T* zc) {
   const T alpha = (T)sc->alpha;
   double m = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -2041,6 +2043,7 @@ __global__ void __launch_bounds__(RED_T) k_cg_update(T* x, T* r, const T* s, con
     const T rv = -alpha * q[i] + r[i];
     r[i] = rv;
     m = fmax(m, fabs((double)rv));
+    if (zc) zc[i] = T(0);
   }
   m = block_reduce<1>(m);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
@@ -2762,11 +2765,12 @@ void launch_cg_direction(hipStream_t st, T* s, const T* z, const CgScalars* sc, 
   k_cg_direction<T><<<nblk(n), TPB, 0, st>>>(s, z, sc, n);
 }
 template <class T>
-void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage) {
+void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage,
+                      T* zclear) {
   // caller passes the partial buffer right after the scalars block
   double* part = reinterpret_cast<double*>(sc + 1);
   const unsigned g = red_grid(n);
-  k_cg_update<T><<<g, RED_T, 0, st>>>(x, r, s, q, sc, n, part);
+  k_cg_update<T><<<g, RED_T, 0, st>>>(x, r, s, q, sc, n, part, zclear);
   k_final<1><<<1, RED_T, 0, st>>>(part, (int)g, &sc->nrm, stage >= 0 ? sc : nullptr, stage);
 }
 void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage) { k_cg_scalars<<<1, 64, 0, st>>>(sc, stage); }
@@ -2867,7 +2871,7 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_dot<T>(hipStream_t, const T*, const T*, int64_t, double*, double*);                       \
   template void launch_linf<T>(hipStream_t, const T*, int64_t, double*, double*);                                \
   template void launch_cg_direction<T>(hipStream_t, T*, const T*, const CgScalars*, int64_t);                    \
-  template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, CgScalars*, int64_t, int);         \
+  template void launch_cg_update<T>(hipStream_t, T*, T*, const T*, const T*, CgScalars*, int64_t, int, T*);    \
   template void launch_project<T>(hipStream_t, const LevelDev&, T*, bool);                                       \
   template void launch_project_dot<T>(hipStream_t, const LevelDev&, T*, const T*, double*, double*, CgScalars*,  \
                                       int);                                                                     \

@@ -132,7 +132,8 @@ void launch_sumsq(hipStream_t st, const double* a, int64_t n, double* partial, d
 template <class T>
 void launch_cg_direction(hipStream_t st, T* s, const T* z, const CgScalars* sc, int64_t n);  // s = beta s + z
 template <class T>
-void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage = -1);
+void launch_cg_update(hipStream_t st, T* x, T* r, const T* s, const T* q, CgScalars* sc, int64_t n, int stage = -1,
+                      T* zclear = nullptr);
 void launch_cg_scalars(hipStream_t st, CgScalars* sc, int stage);  // 0: after rho, 1: after sq, 2: end
 // (the reductions above take `sc, stage` to run a stage in their last kernel)
 // projections

@@ -483,10 +483,10 @@ void Solver::bottom(const T* b, T* u) {
 // FP32 cycle whose explicit levels (and bottom) run in FP64: the level
 // handed over is restricted in FP32 and widened, and widened corrections are
 // rounded back before the FP32 prolongation
-void Solver::vcycle_coarse64(const float* b0, float* z) {
+void Solver::vcycle_coarse64(const float* b0, float* z, bool z_is_zero) {
   const int L = lay_.nlev;
   const int ex = lay_.explicit_level;
-  launch_zero<float>(st_, z, lv_[0].n3);
+  if (!z_is_zero) launch_zero<float>(st_, z, lv_[0].n3);
   for (int l = 0; l < L - 1; ++l) {
     const bool d = l >= ex, dn = l + 1 >= ex;
     const bool sm = (l == 0 || l >= ex);
@@ -530,11 +530,11 @@ void Solver::vcycle_coarse64(const float* b0, float* z) {
 }
 
 template <class T>
-void Solver::vcycle(const T* b0, T* z) {
+void Solver::vcycle(const T* b0, T* z, bool z_is_zero) {
   const int L = lay_.nlev;
   const int ex = lay_.explicit_level;
   if (sizeof(T) == 4 && coarse64_ && lay_.use_mg && L > ex) {
-    vcycle_coarse64(reinterpret_cast<const float*>(b0), reinterpret_cast<float*>(z));
+    vcycle_coarse64(reinterpret_cast<const float*>(b0), reinterpret_cast<float*>(z), z_is_zero);
     return;
   }
   if (!lay_.use_mg) {
@@ -545,7 +545,7 @@ void Solver::vcycle(const T* b0, T* z) {
     bottom<T>(b0, z);
     return;
   }
-  launch_zero<T>(st_, z, lv_[0].n3);
+  if (!z_is_zero) launch_zero<T>(st_, z, lv_[0].n3);
   for (int l = 0; l < L - 1; ++l) {
     T* u = l == 0 ? z : lv_[l].u<T>();
     const T* b = l == 0 ? b0 : lv_[l].b<T>();
@@ -583,6 +583,7 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
   double* part = part_.as<double>();
   launch_zero<T>(st_, x, n);
   launch_zero<T>(st_, s, n);
+  launch_zero<T>(st_, z, n);
   launch_project<T>(st_, R0.d, r, true);
   launch_cg_scalars(st_, sc, 3);
   launch_linf<T>(st_, r, n, part, &sc->nrm);
@@ -611,13 +612,13 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
       break;
     }
     if (it == maxit) break;
-    vcycle<T>(r, z);
+    vcycle<T>(r, z, true);  // z was cleared before the loop / by the last update
     // the scalar stages (beta, alpha, rho_old) run in the last kernel of
     // the reduction they follow
     launch_project_dot<T>(st_, R0.d, z, r, part, &sc->rho, sc, 0);
     launch_cg_direction<T>(st_, s, z, sc, n);
     launch_apply_dot<T>(st_, R0.d, E<T>(), dxT<T>(), s, q, apart_.as<double>(), part, &sc->sq, sc, 1);
-    launch_cg_update<T>(st_, x, r, s, q, sc, n, 2);
+    launch_cg_update<T>(st_, x, r, s, q, sc, n, 2, z);
   }
   *iters = it;
   return rc;
@@ -1024,7 +1025,7 @@ double Solver::bench_kernel(int op, int level, int precision, int arg, int reps)
   return (double)ms * 1e-3 / reps;
 }
 
-template void Solver::vcycle<float>(const float*, float*);
-template void Solver::vcycle<double>(const double*, double*);
+template void Solver::vcycle<float>(const float*, float*, bool);
+template void Solver::vcycle<double>(const double*, double*, bool);
 
 }  // namespace spg

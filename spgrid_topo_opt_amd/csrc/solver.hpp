@@ -90,8 +90,9 @@ class Solver {
   double device_bytes() const { return dev_bytes_; }
 
  private:
-  template <class T> void vcycle(const T* b0, T* z);
-  void vcycle_coarse64(const float* b0, float* z);
+  // z = M b0; `z_is_zero` says the caller has already cleared z
+  template <class T> void vcycle(const T* b0, T* z, bool z_is_zero = false);
+  void vcycle_coarse64(const float* b0, float* z, bool z_is_zero);
   template <class T> void smooth(int l, T* u, const T* b, bool forward);
   template <class T> void level_residual(int l, const T* u, const T* b, T* r);
   template <class T> void bottom(const T* b, T* u);
