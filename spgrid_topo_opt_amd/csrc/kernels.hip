@@ -2621,7 +2621,7 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
   } else if (kind == 3) {
     // SPG_GS_QB blocks per workgroup (4, 8 or 16); 256^3 shell, FP32 / FP64
-    // sweep: 4 -> 318-327 / 478-480 us, 8 -> 322 / 483 us, 16 -> 349 / 552 us
+    // sweep: 4 -> 284-308 / 497-509 us, 8 -> 279-281 / 468 us, 16 -> 300 / 489 us
     const char* q = std::getenv("SPG_GS_QB");
     const int qb = q ? std::atoi(q) : 8;
     if (qb == 4)
