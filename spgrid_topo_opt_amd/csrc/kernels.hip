@@ -2528,8 +2528,9 @@ static bool launch_bnd0_phase(hipStream_t st, const LevelDev& L, T dx, T* u, con
 }
 // level-0 boundary phase as one launch over several CUs; false if the
 // tables are absent or a cooperative launch is refused (the caller then runs
-// another path; `loud` reports a refused launch on stderr).  SPG_COOP_WG caps the workgroup count (tests use it to make
-// a workgroup take several node groups per colour step).
+// another path; `loud` reports a refused launch on stderr).  SPG_COOP_WG
+// caps the workgroup count (tests use it to make a workgroup take several
+// node groups per colour step).
 template <class T>
 static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, const T* b, int sweeps, bool forward,
                              bool loud) {
@@ -2565,7 +2566,6 @@ static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, cons
   unsigned* coop = L.coop;
   unsigned long long lim = limit;
   int nn = n, sw = sweeps;
-  void* args[] = {&nn, &o, &ord, &sw, &list, &flags, &bE, &cn, &bD, &bt, &dx, &ub, &u, &coop, &lim};
   // An ordinary launch by default: the solver's stream runs one kernel at a
   // time, so the G <= 32 small workgroups all start at once on the empty
   // device (256 CUs).  The cooperative launch API guarantees that even
@@ -2578,6 +2578,7 @@ static bool launch_bnd0_coop(hipStream_t st, const LevelDev& L, T dx, T* u, cons
     k_bnd0_coop<T><<<G, BC_THREADS, 0, st>>>(nn, o, ord, sw, list, flags, bE, cn, bD, bt, dx, ub, u, coop, lim);
     return true;
   }
+  void* args[] = {&nn, &o, &ord, &sw, &list, &flags, &bE, &cn, &bD, &bt, &dx, &ub, &u, &coop, &lim};
   const hipError_t e = hipLaunchCooperativeKernel((const void*)k_bnd0_coop<T>, dim3(G), dim3(BC_THREADS), args, 0, st);
   if (e == hipSuccess) return true;
   (void)hipGetLastError();
