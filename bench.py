@@ -92,12 +92,15 @@ def main():
 
     # kernel throughput first: it does not depend on how the solve converges
     nn = s.info()["active_nodes"]
-    t_ap32 = s.bench_apply(0, args.apply_reps)
+    t_ap32 = s.bench_apply(0, args.apply_reps)  # the PCG operator: FP32, difference form
     t_ap64 = s.bench_apply(1, args.apply_reps)
-    # the same FP32 apply with FP32 row sums (the PCG operator sums in FP64)
+    # the direct form with FP64 row sums, then with plain FP32 row sums
+    os.environ["SPG_DIFF_SUMS"] = "0"
+    t_ap32d = s.bench_apply(0, args.apply_reps)
     os.environ["SPG_ACC64_MODES"] = "0"
     t_ap32f = s.bench_apply(0, args.apply_reps)
     del os.environ["SPG_ACC64_MODES"]
+    del os.environ["SPG_DIFF_SUMS"]
     t_gs32 = s.bench_kernel(_lib.OP_GS_SWEEP, 0, 0, 0, 20)  # 8 colour launches
     t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
     t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
@@ -183,6 +186,7 @@ def main():
         "apply_mdof_s_fp32": 3 * nn / t_ap32 / 1e6,
         "apply_mdof_s_fp64": 3 * nn / t_ap64 / 1e6,
         "apply_us_fp32": t_ap32 * 1e6,
+        "apply_us_fp32_fp64_sums": t_ap32d * 1e6,
         "apply_us_fp32_fp32_sums": t_ap32f * 1e6,
         "apply_mdof_s_fp32_fp32_sums": 3 * nn / t_ap32f / 1e6,
         "apply_us_fp64": t_ap64 * 1e6,
@@ -190,13 +194,13 @@ def main():
         "vcycle_ms_fp64": t_vc64 * 1e3,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_apply_lds<float, double, 2> (level-0 stiffness apply of the PCG operator: FP32 in/out, "
-                      "FP64 row sums)",
+            "kernel": "k_apply_lds<float, float, 4> (level-0 stiffness apply of the PCG operator: FP32 in/out, "
+                      "FP32 row sums in difference form)",
             "achieved": gbs32,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_apply_lds<float, double, 2>", nn),
+            "traffic": measured_traffic("k_apply_lds<float, float, 4>", nn),
         },
         "roofline_gs_sweep": {
             "bound": "hbm",

@@ -165,6 +165,7 @@ int spg_level_nodes(spg_solver *s, int32_t level, int32_t *xyz /*host [nb*64*3]*
 #define SPG_OP_STENCIL_RES 6  /* out = P(b - A in) on an explicit level        */
 #define SPG_OP_VCYCLE 7       /* out = V-cycle(b) (level 0)                    */
 #define SPG_OP_BOTTOM 8       /* out = A^-1 b on the bottom level              */
+#define SPG_OP_OPERATOR 9     /* out = P K in (level 0): the Krylov operator   */
 /* Run one operator on host vectors (copied to/from the device);
  * precision 0 = FP32 kernels, 1 = FP64 kernels; bit 0 of `forward` selects
  * the GS colour order, bits 8+ the number of boundary sweeps (0 = one).    */

@@ -23,7 +23,8 @@ SPG_E_HIP = -5
 SPG_E_NAN = -6
 SPG_E_VERSION = -7
 
-OP_APPLY, OP_RESIDUAL, OP_GS_SWEEP, OP_GS_BOUNDARY, OP_RESTRICT, OP_PROLONGATE, OP_STENCIL_RES, OP_VCYCLE, OP_BOTTOM = range(9)
+OP_APPLY, OP_RESIDUAL, OP_GS_SWEEP, OP_GS_BOUNDARY, OP_RESTRICT, OP_PROLONGATE, OP_STENCIL_RES, OP_VCYCLE, OP_BOTTOM, \
+    OP_OPERATOR = range(10)
 
 # exported C symbols (kept in sync with include/spgrid_topo_opt_amd.h)
 EXPORTS = [

@@ -202,7 +202,8 @@ static constexpr int AP_WAVES = 4;
 // sums (FP32 vectors may be summed in FP64, which gfx950 issues at the same
 // rate as FP32 on the vector ALUs)
 // MODE 0: out = K in; 1: out = P (b - K in); 2: out = P K in; 3: as 2, and
-// each block's partial sum of in . out goes to dpart[block].
+// each block's partial sum of in . out goes to dpart[block]; 4 and 5: as 2
+// and 3 in difference form (below).
 template <class T, class A, int MODE>
 __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* __restrict__ nbr,
                                                              const uint8_t* __restrict__ flags,
@@ -269,6 +270,19 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
 #pragma unroll
     for (int c = 0; c < 3; ++c) uu[s][c] = (A)hu[w][c][h];
   }
+  if (MODE >= 4) {
+    // difference form: every element's 3x3 block rows sum to zero (rigid
+    // translations), so K u = sum_m K_nm (u_m - u_n).  Where u is smooth the
+    // differences are small and the row sums no longer cancel large terms,
+    // so FP32 sums keep the accuracy the Krylov operator needs.
+#pragma unroll
+    for (int s = 0; s < 27; ++s)
+      if (s != 13)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) uu[s][c] -= uu[13][c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uu[13][c] = A(0);
+  }
   A f0 = 0, f1 = 0, f2 = 0;
 #pragma unroll
   for (int I = 0; I < 8; ++I) {
@@ -307,9 +321,9 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
       const T v = MODE >= 2 ? f[c] : (kPre ? pb[c] : b[vidx(blk, c, lane)]) - f[c];
       const T o = (fl & (2 << c)) ? T(0) : v;
       out[vidx(blk, c, lane)] = o;
-      if (MODE == 3) d += (double)hu[w][c][(li + 1) * 36 + (lj + 1) * 6 + (lk + 1)] * (double)o;
+      if (MODE == 3 || MODE == 5) d += (double)hu[w][c][(li + 1) * 36 + (lj + 1) * 6 + (lk + 1)] * (double)o;
     }
-    if (MODE == 3) {
+    if (MODE == 3 || MODE == 5) {
       // one partial per wavefront (= per block)
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
@@ -2381,13 +2395,15 @@ __global__ void k_set_entries(double* v, const int* idx, const double* val, int6
 }
 
 // ======================================================== launch wrappers
-// The FP32 operator of the Krylov recurrence (apply mode 2, q = P K s) sums
-// its rows in FP64 with FP32 in / out: FP32 accumulation of the 576-term rows
-// loses accuracy to cancellation on smooth search directions and cost the
-// FP32 MG-PCG 23 instead of 18 iterations on the 256^3 shell (the FP64 solve
-// and the reference's float mode take 18).  The V-cycle residual (mode 1)
-// keeps FP32 sums: the preconditioner tolerates them.  SPG_ACC64_MODES
-// (bit m = apply mode m) overrides the choice.
+// The FP32 operator of the Krylov recurrence (apply mode 2, q = P K s):
+// direct FP32 accumulation of the 576-term rows loses accuracy to
+// cancellation on smooth search directions and cost the FP32 MG-PCG 23
+// instead of 18 iterations on the 256^3 shell (the FP64 solve and the
+// reference's float mode take 18).  Two cures: rows summed in FP64
+// (SPG_ACC64_MODES, bit m = apply mode m; the default when SPG_DIFF_SUMS=0),
+// or the difference form with FP32 sums (the default: 18 iterations, and
+// 105-110 instead of 160 us per apply).  The V-cycle residual (mode 1)
+// keeps plain FP32 sums: the preconditioner tolerates them.
 static int acc64_modes() {
   const char* e = std::getenv("SPG_ACC64_MODES");
   return e ? std::atoi(e) : 4;
@@ -2403,10 +2419,22 @@ static void launch_apply_t(hipStream_t st, const LevelDev& L, const T* E, T dx, 
   else
     k_apply_lds<T, A, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
 }
+// The FP32 Krylov operator P K s (mode 2) sums in FP32 in difference form;
+// SPG_DIFF_SUMS=0 selects the direct form, summed in FP64 by default
+static bool diff_sums() {
+  const char* e = std::getenv("SPG_DIFF_SUMS");
+  return !(e && std::atoi(e) == 0);
+}
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {
-  if (sizeof(T) == 4 && ((acc64_modes() >> mode) & 1)) launch_apply_t<T, double>(st, L, E, dx, in, b, out, mode);
-  else launch_apply_t<T, T>(st, L, E, dx, in, b, out, mode);
+  if (sizeof(T) == 4 && mode == 2 && diff_sums()) {
+    const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
+    k_apply_lds<T, T, 4><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
+  } else if (sizeof(T) == 4 && ((acc64_modes() >> mode) & 1)) {
+    launch_apply_t<T, double>(st, L, E, dx, in, b, out, mode);
+  } else {
+    launch_apply_t<T, T>(st, L, E, dx, in, b, out, mode);
+  }
 }
 int64_t apply_dot_partials(int64_t nb) { return nb > RED_MAX_WG ? nb : RED_MAX_WG; }
 template <class T>
@@ -2415,7 +2443,9 @@ void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const
   // 256^3 shell, 18 iterations: mixed 42.8 vs 43.1 ms, FP64 60.0 vs 60.5 ms
   // against the apply followed by a separate dot
   const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
-  if (sizeof(T) == 4 && ((acc64_modes() >> 2) & 1))
+  if (sizeof(T) == 4 && diff_sums())
+    k_apply_lds<T, T, 5><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
+  else if (sizeof(T) == 4 && ((acc64_modes() >> 2) & 1))
     k_apply_lds<T, double, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
   else
     k_apply_lds<T, T, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);

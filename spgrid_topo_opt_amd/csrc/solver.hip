@@ -831,13 +831,14 @@ void Solver::run_kernel(int op, int level, int precision, int forward, const dou
     DBuf A, Bv, O;
     switch (op) {
       case SPG_OP_APPLY:
-      case SPG_OP_RESIDUAL: {
+      case SPG_OP_RESIDUAL:
+      case SPG_OP_OPERATOR: {
         if (level != 0) throw SpgError(SPG_E_INVALID, "apply is a level-0 operator");
         put(A, in, nb);
         if (op == SPG_OP_RESIDUAL) put(Bv, b, nb);
         O.alloc(nb * 192 * sizeof(T));
         launch_apply<T>(st_, lv_[0].d, E<T>(), dxT<T>(), A.as<T>(), Bv.as<T>(), O.as<T>(),
-                        op == SPG_OP_APPLY ? 0 : 1);
+                        op == SPG_OP_APPLY ? 0 : (op == SPG_OP_RESIDUAL ? 1 : 2));
         get(O.as<T>(), nb);
         break;
       }
