@@ -2399,11 +2399,13 @@ __global__ void k_set_entries(double* v, const int* idx, const double* val, int6
 // direct FP32 accumulation of the 576-term rows loses accuracy to
 // cancellation on smooth search directions and cost the FP32 MG-PCG 23
 // instead of 18 iterations on the 256^3 shell (the FP64 solve and the
-// reference's float mode take 18).  Two cures: rows summed in FP64
-// (SPG_ACC64_MODES, bit m = apply mode m; the default when SPG_DIFF_SUMS=0),
-// or the difference form with FP32 sums (the default: 18 iterations, and
-// 105-110 instead of 160 us per apply).  The V-cycle residual (mode 1)
-// keeps plain FP32 sums: the preconditioner tolerates them.
+// reference's float mode take 18).  The rows are therefore summed in FP64
+// (SPG_ACC64_MODES, bit m = apply mode m).  The difference form with FP32
+// sums (SPG_DIFF_SUMS=1) removes the cancellation of translations only: it
+// keeps 18 iterations on the shell at 118 instead of 152-160 us per apply,
+// but the 2048-wide ellipsoid, whose displacement is mostly rotation, needs
+// 44 iterations in 3 passes instead of 32 in 2.  The V-cycle residual
+// (mode 1) keeps plain FP32 sums: the preconditioner tolerates them.
 static int acc64_modes() {
   const char* e = std::getenv("SPG_ACC64_MODES");
   return e ? std::atoi(e) : 4;
@@ -2419,11 +2421,11 @@ static void launch_apply_t(hipStream_t st, const LevelDev& L, const T* E, T dx, 
   else
     k_apply_lds<T, A, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
 }
-// The FP32 Krylov operator P K s (mode 2) sums in FP32 in difference form;
-// SPG_DIFF_SUMS=0 selects the direct form, summed in FP64 by default
+// SPG_DIFF_SUMS=1: the FP32 Krylov operator P K s (mode 2) in difference
+// form with FP32 sums instead of the direct form with FP64 sums
 static bool diff_sums() {
   const char* e = std::getenv("SPG_DIFF_SUMS");
-  return !(e && std::atoi(e) == 0);
+  return e && std::atoi(e) == 1;
 }
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {

@@ -96,9 +96,10 @@ def test_apply_and_residual(blocky, prec, tol):
 
 
 def test_fp32_operator_difference_form_on_smooth_fields(blocky, monkeypatch):
-    """the FP32 Krylov operator P K u sums K (u_m - u_n) (element rows sum to
-    zero): on a smooth field, where the direct FP32 row sums cancel large
-    terms, its error against the FP64 apply is far below the direct form's"""
+    """the FP32 Krylov operator P K u on a smooth field, where direct FP32 row
+    sums cancel large terms: both cures (FP64 row sums, the default; the
+    difference form K (u_m - u_n) with FP32 sums, SPG_DIFF_SUMS=1) stay far
+    below the direct FP32 form's error against the FP64 apply"""
     P = blocky
     gx, gf = P.g.level_nodes(0)
     x = gx.astype(np.float64)
@@ -108,14 +109,16 @@ def test_fp32_operator_difference_form_on_smooth_fields(blocky, monkeypatch):
     u = 1.0 + 1e-3 * np.sin(x / 7.0 + np.array([0.0, 1.0, 2.0]))
     u = u.astype(np.float32).astype(np.float64)  # the FP64 reference sees the same FP32 input
     want = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=1)
+    got_fp64sums = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)  # the default
+    monkeypatch.setenv("SPG_DIFF_SUMS", "1")
     got_diff = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)
     monkeypatch.setenv("SPG_DIFF_SUMS", "0")
     monkeypatch.setenv("SPG_ACC64_MODES", "0")
     got_direct = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)
-    e_diff, e_direct = rel(got_diff, want), rel(got_direct, want)
-    print(f"smooth field: difference form {e_diff:.2e}, direct FP32 sums {e_direct:.2e}")
-    assert e_diff < 1e-4
-    assert e_diff < 0.05 * e_direct
+    e64, e_diff, e_direct = rel(got_fp64sums, want), rel(got_diff, want), rel(got_direct, want)
+    print(f"smooth field: FP64 sums {e64:.2e}, difference form {e_diff:.2e}, direct FP32 sums {e_direct:.2e}")
+    assert e64 < 1e-5 and e_diff < 1e-4
+    assert e64 < 0.05 * e_direct and e_diff < 0.05 * e_direct
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
