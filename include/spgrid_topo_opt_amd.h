@@ -81,6 +81,19 @@ typedef struct {
                               PCG iteration, one per line (CG/
                               CONJUGATE_GRADIENT.cpp:34-35); 1 stdout, as the
                               reference, 2 stderr, 0 off                     */
+  int32_t warm_start;      /* the reference's warm start
+                              (solver/SPGrid_CPU_Solver.cpp:241-285):
+                              0 solve from u_D; 1 start from the solver's
+                              last solution (or the guess given with
+                              spg_set_initial_guess_device) if its residual
+                              is below reference_linf, else from u_D;
+                              2 always start from it (forced_reuse)        */
+  int32_t reserved0;       /* zero                                          */
+  double reference_linf;   /* warm starts: the stored zero-guess residual
+                              of the first solve (state.initial_residual_linf,
+                              SPGrid_CPU_Solver.cpp:318-326); the target is
+                              tolerance * reference_linf.  <= 0: the
+                              zero-guess residual of this problem           */
 } spg_solve_params;
 
 typedef struct {
@@ -91,6 +104,9 @@ typedef struct {
   int64_t active_nodes, active_cells, gpu_blocks;
   int64_t bottom_dofs;
   double hbm_bytes;        /* device memory held by the solver              */
+  double r_start_linf;     /* FP64 residual (L_inf) of the starting guess   */
+  int32_t warm_reused;     /* 1: the solve started from the previous u      */
+  int32_t reserved1;
 } spg_info;
 
 const char *spg_last_error(void);
@@ -127,6 +143,13 @@ int spg_solve(spg_solver *s, const spg_solve_params *prm, double *u_out4 /*host,
 /* Device-resident views of the last solution (FP64, [nblocks*64*3] in input
  * block order): copies into a device buffer owned by the caller.            */
 int spg_get_displacement_device(spg_solver *s, double *u_dev /*device [nblocks*64*3]*/);
+
+/* Starting guess for the next warm-started solve (spg_solve_params.
+ * warm_start != 0), in input block order (device [nblocks*64*3]): the
+ * previous displacement carried over to a rebuilt solver, as the
+ * reference's kept SPGrid u channel is (SPGrid_CPU_Solver.cpp:58-75,
+ * 241-285).  Dirichlet entries are reset to u_D by the solve.               */
+int spg_set_initial_guess_device(spg_solver *s, const double *u_dev);
 
 /* Adjoint solve K lambda = g for a right-hand side given per input block
  * node (device [nblocks*64*3]); result in the same layout.                  */
@@ -191,18 +214,27 @@ int spg_bench_kernel(spg_solver *s, int32_t op, int32_t level, int32_t precision
 /* Sensitivity filter smooth_dc (topo_opt/spgrid_topo_opt.cpp:527-574):
  *   out(x) = sum_o w(o) dc(x+o) rho(x+o) [x+o inside]
  *            / (sum_o w(o) [x+o inside] * (rho(x) + 1e-3)),
- *   w(o) = max(0, r - |o|), |o_i| <= floor(r) <= 3; 0 outside the container. */
+ *   w(o) = max(0, r - |o|), |o_i| <= floor(r) <= 3.  Every cell takes the
+ *   formula, inside the container or not, as smooth_dc writes every cell of
+ *   the fat blocks; a cell with no inside cell in its window gets 0 (the
+ *   reference computes 0/0 there).                                           */
 int spg_filter_sensitivity(int64_t nblocks, const int32_t *nbr, const uint8_t *inside, const double *dc,
                            const double *rho, double filter_size, double *out);
 
 /* One trial of the optimality-criterion bisection
- * (topo_opt/optimizer/optimality_criterion.cpp:76-118) at multiplier mid:
- *   new = clamp(clamp(rho sqrt(max(0,-dc)/mid), rho-step, rho+step), min_density, 1)
- * on the cells with mask != 0, rho elsewhere.  *volume = sum of new over the
- * masked cells (a deterministic FP64 sum); with out != NULL the new
- * densities are written there and *change = max |new - rho|.               */
+ * (topo_opt/optimizer/optimality_criterion.cpp:76-118) at multiplier mid.
+ * mask bit 0: the cell is inside the container and is updated; bit 1: the
+ * cell has a fixed density (flags.get_fixed_density()).  Updated cells take
+ *   new = clamp(clamp(rho sqrt(max(0,-dc-dc_shift)/mid), rho-step, rho+step),
+ *               min_density, 1)
+ * or fixed_density for fixed cells; other cells keep rho.  dc_shift is
+ * min(0, min -dc over the container) under smart_oc, else 0.  *volume = the
+ * sum of new over updated cells (fixed ones left out when exclude_fixed),
+ * a deterministic FP64 sum; with out != NULL the new densities are written
+ * there and *change = max |new - rho| over updated cells.                   */
 int spg_oc_trial(int64_t n, const uint8_t *mask, const double *rho, const double *dc, double mid, double step,
-                 double min_density, double *out, double *volume, double *change);
+                 double min_density, double fixed_density, int32_t exclude_fixed, double dc_shift, double *out,
+                 double *volume, double *change);
 
 #ifdef __cplusplus
 }

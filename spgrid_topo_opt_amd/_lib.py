@@ -32,7 +32,7 @@ EXPORTS = [
     "spg_destroy", "spg_update_density", "spg_set_forces", "spg_solve", "spg_get_displacement_device",
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
     "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel", "spg_get_info",
-    "spg_update_density_device", "spg_filter_sensitivity", "spg_oc_trial",
+    "spg_update_density_device", "spg_filter_sensitivity", "spg_oc_trial", "spg_set_initial_guess_device",
 ]
 
 
@@ -66,6 +66,9 @@ class SolveParams(C.Structure):
         ("max_passes", C.c_int32),
         ("precision", C.c_int32),
         ("print_residuals", C.c_int32),
+        ("warm_start", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("reference_linf", C.c_double),
     ]
 
 
@@ -85,6 +88,9 @@ class Info(C.Structure):
         ("gpu_blocks", C.c_int64),
         ("bottom_dofs", C.c_int64),
         ("hbm_bytes", C.c_double),
+        ("r_start_linf", C.c_double),
+        ("warm_reused", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
     def as_dict(self):
@@ -121,6 +127,7 @@ def lib():
         L.spg_set_forces.argtypes = [vp, C.c_int64, ip, dp]
         L.spg_solve.argtypes = [vp, C.POINTER(SolveParams), dp, C.POINTER(Info)]
         L.spg_get_displacement_device.argtypes = [vp, vp]
+        L.spg_set_initial_guess_device.argtypes = [vp, vp]
         L.spg_solve_adjoint_device.argtypes = [vp, C.POINTER(SolveParams), vp, vp, C.POINTER(Info)]
         L.spg_sensitivity_device.argtypes = [vp, vp, vp, vp, dp]
         L.spg_num_levels.argtypes = [vp, ip]
@@ -133,7 +140,8 @@ def lib():
         L.spg_get_info.argtypes = [vp, C.POINTER(Info)]
         L.spg_update_density_device.argtypes = [vp, vp]
         L.spg_filter_sensitivity.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, vp]
-        L.spg_oc_trial.argtypes = [C.c_int64, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, dp, dp]
+        L.spg_oc_trial.argtypes = [C.c_int64, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                   C.c_double, vp, dp, dp]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -219,9 +227,12 @@ class Solver:
         _check(lib().spg_update_density_device(self.h, C.c_void_p(ptr)))
 
     def solve(self, tolerance=1e-6, inner_tolerance=0.0, max_iterations=500, max_passes=1, precision=0,
-              want_output=True, raise_on_error=True, print_residuals=0):
-        """print_residuals: 1 prints ||r||_inf of every PCG iteration to stdout, 2 to stderr"""
-        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, print_residuals)
+              want_output=True, raise_on_error=True, print_residuals=0, warm_start=0, reference_linf=0.0):
+        """print_residuals: 1 prints ||r||_inf of every PCG iteration to stdout, 2 to stderr.
+        warm_start: 0 from u_D, 1 from the last solution / set guess if its
+        residual is below reference_linf (or the zero-guess residual), 2 always."""
+        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, print_residuals,
+                        warm_start, 0, reference_linf)
         info = Info()
         out = np.zeros((self.nblocks, 64, 4)) if want_output else None
         rc = lib().spg_solve(self.h, C.byref(p), _dp(out), C.byref(info))
@@ -243,9 +254,13 @@ class Solver:
     def displacement_device(self, ptr):
         _check(lib().spg_get_displacement_device(self.h, C.c_void_p(ptr)))
 
+    def set_initial_guess_device(self, ptr):
+        """starting guess of the next warm-started solve (input block order, device)"""
+        _check(lib().spg_set_initial_guess_device(self.h, C.c_void_p(ptr)))
+
     def solve_adjoint_device(self, g_ptr, lam_ptr, tolerance=1e-6, inner_tolerance=0.0, max_iterations=500,
                              max_passes=30, precision=0):
-        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, 0)
+        p = SolveParams(tolerance, inner_tolerance, max_iterations, max_passes, precision, 0, 0, 0, 0.0)
         info = Info()
         _check(lib().spg_solve_adjoint_device(self.h, C.byref(p), C.c_void_p(g_ptr), C.c_void_p(lam_ptr),
                                               C.byref(info)))
@@ -317,11 +332,14 @@ def filter_sensitivity(nblocks, nbr_ptr, inside_ptr, dc_ptr, rho_ptr, filter_siz
                                         C.c_void_p(rho_ptr), filter_size, C.c_void_p(out_ptr)))
 
 
-def oc_trial(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, out_ptr=None):
-    """one OC trial over device arrays; returns (volume, max change)"""
+def oc_trial(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, out_ptr=None, fixed_density=1.0,
+             exclude_fixed=True, dc_shift=0.0):
+    """one OC trial over device arrays (mask bit 0 updated, bit 1 fixed);
+    returns (volume, max change)"""
     vol, ch = C.c_double(0.0), C.c_double(0.0)
     _check(lib().spg_oc_trial(n, C.c_void_p(mask_ptr), C.c_void_p(rho_ptr), C.c_void_p(dc_ptr), mid, step,
-                              min_density, C.c_void_p(out_ptr) if out_ptr else None, C.byref(vol), C.byref(ch)))
+                              min_density, fixed_density, 1 if exclude_fixed else 0, dc_shift,
+                              C.c_void_p(out_ptr) if out_ptr else None, C.byref(vol), C.byref(ch)))
     return vol.value, ch.value
 
 

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/spgrid_topo_opt_amd.h"
@@ -14,6 +15,17 @@
 
 using namespace spg;
 
+// What a kept fem_solve state carries to the next call besides the solver:
+// the last displacement per input block (the reference keeps its SPGrid u
+// channel, SPGrid_CPU_Solver.cpp:389-404) and the zero-guess residual of
+// the first solve (state.initial_residual_linf, :318-326).
+struct WarmState {
+  std::vector<int32_t> base;  // [nb*3] input blocks of the last solve
+  std::vector<double> u;      // [nb*64*3] their displacement
+  double initial_linf = 0.0;
+  bool valid = false;
+};
+
 struct spg_solver {
   std::unique_ptr<Solver> s;
   // inputs kept for reuse checks and density updates
@@ -21,6 +33,7 @@ struct spg_solver {
   std::vector<int32_t> dcoord, daxis;
   std::vector<double> dvalue;
   spg_config cfg;
+  WarmState warm;
 };
 
 static thread_local std::string g_err;
@@ -160,6 +173,14 @@ int spg_get_displacement_device(spg_solver* s, double* u_dev) {
   API_CATCH
 }
 
+int spg_set_initial_guess_device(spg_solver* s, const double* u_dev) {
+  API_TRY
+  if (!s || !u_dev) throw SpgError(SPG_E_INVALID, "null argument");
+  s->s->set_initial_guess_in3(u_dev);
+  return SPG_OK;
+  API_CATCH
+}
+
 int spg_solve_adjoint_device(spg_solver* s, const spg_solve_params* prm, const double* g_dev, double* lam_dev,
                              spg_info* info) {
   API_TRY
@@ -284,23 +305,44 @@ static bool same_topology(const spg_solver* h, const FemInputs& in, const std::v
          c.smooth_iters == in.pre_and_post_smoothing_iter && c.boundary_iters == in.get_int("boundary_smoothing", 5);
 }
 
+// The previous displacement mapped onto a new block list by block
+// coordinates (blocks new to the list start at zero).
+static std::vector<double> carry_over(const WarmState& w, const std::vector<int32_t>& base) {
+  std::unordered_map<uint64_t, size_t> prev;
+  const size_t np = w.base.size() / 3;
+  prev.reserve(np * 2);
+  for (size_t b = 0; b < np; ++b) prev.emplace(bkey(w.base[3 * b], w.base[3 * b + 1], w.base[3 * b + 2]), b);
+  const size_t nb = base.size() / 3;
+  std::vector<double> u(nb * 192, 0.0);
+  for (size_t b = 0; b < nb; ++b) {
+    auto it = prev.find(bkey(base[3 * b], base[3 * b + 1], base[3 * b + 2]));
+    if (it != prev.end()) std::memcpy(&u[b * 192], &w.u[it->second * 192], 192 * sizeof(double));
+  }
+  return u;
+}
+
 // Drop-in for `fem_solve` (topo_opt/fem_interface.h:11; solver/SPGrid_CPU_Solver.cpp:26-409).
 void fem_solve(void* input, void* output) {
   FemOutputs out;
   out.success = false;
   if (!output) return;
-  spg_solver* keep = nullptr;
+  // the kept state: returned to the caller when keep_state is set (also
+  // after a failed solve, so the caller can still release it), freed
+  // otherwise (SPGrid_CPU_Solver.cpp:389-408)
+  std::unique_ptr<spg_solver> h;
+  bool keep = false;
   try {
     FemInputs in;
     WireReader r(input);
     read_inputs(r, in);
+    h.reset(static_cast<spg_solver*>(in.get_state_ptr()));
+    keep = in.get_int("keep_state", 0) != 0;
     for (int d = 0; d < 3; ++d) out.displacements.resolution[d] = in.resolution[d];
     if (!in.use_density_only) {
       // SPGrid_CPU_Solver.cpp:131-135
       out.info = "Unsupported mode! (mu/lambda grids are not supported; use_density_only must be true)";
       out.error_code = SPG_E_INVALID;
-      emit(out, output);
-      return;
+      throw SpgError(SPG_E_INVALID, out.info);
     }
     const size_t nb = in.density.blocks.size();
     std::vector<int32_t> base(nb * 3);
@@ -328,16 +370,34 @@ void fem_solve(void* input, void* output) {
     cfg.smooth_iters = in.pre_and_post_smoothing_iter;
     cfg.boundary_iters = in.get_int("boundary_smoothing", 5);
     cfg.device = in.get_int("device", 0);
-    // solver state reuse (fem_interface.h:193-209, SPGrid_CPU_Solver.cpp:58-75,389-408)
-    spg_solver* h = static_cast<spg_solver*>(in.get_state_ptr());
-    if (h && same_topology(h, in, base)) {
+    // solver state reuse (fem_interface.h:193-209, SPGrid_CPU_Solver.cpp:58-75):
+    // a state pointer means a warm start.  The kept hierarchy serves when the
+    // blocks, Dirichlet data and settings are the same and no cell switches
+    // on or off; otherwise the solver is rebuilt (the reference re-flags
+    // every call, :102-145) and the previous u is carried over by block.
+    const bool warm = h != nullptr;
+    WarmState prev;
+    if (h) prev = std::move(h->warm);
+    bool rebuilt = true;
+    if (h && same_topology(h.get(), in, base) && h->s->active_set_matches(dens.data())) {
       h->s->update_density(dens.data());
+      rebuilt = false;
     } else {
-      delete h;
-      h = nullptr;
+      h.reset();
+      spg_solver* nh = nullptr;
       int rc = spg_create(&cfg, (int64_t)nb, base.data(), dens.data(), (int64_t)da.size(), dc.data(), da.data(),
-                          dv.data(), &h);
+                          dv.data(), &nh);
       if (rc != SPG_OK) throw SpgError(rc, g_err);
+      h.reset(nh);
+    }
+    if (warm && rebuilt && prev.valid) {
+      const std::vector<double> g = carry_over(prev, base);
+      double* gd = nullptr;
+      if (hipMalloc(&gd, g.size() * sizeof(double)) != hipSuccess) throw SpgError(SPG_E_HIP, "hipMalloc failed");
+      std::unique_ptr<double, void (*)(double*)> guard(gd, [](double* p) { (void)hipFree(p); });
+      if (hipMemcpy(gd, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        throw SpgError(SPG_E_HIP, "hipMemcpy failed");
+      h->s->set_initial_guess_in3(gd);
     }
     std::vector<int32_t> fc(in.forces.size() * 3);
     std::vector<double> fv(in.forces.size() * 3);
@@ -355,6 +415,9 @@ void fem_solve(void* input, void* output) {
     prm.precision = in.get_int("fp64_solve", 0) ? 1 : 0;
     prm.inner_tolerance = in.get_double("inner_tolerance", 0.0);
     prm.print_residuals = in.krylov.print_residuals ? 1 : 0;
+    // warm start: reuse u if forced or if it beats the stored first residual
+    prm.warm_start = warm ? (in.get_int("forced_reuse", 0) ? 2 : 1) : 0;
+    prm.reference_linf = warm && prev.valid ? prev.initial_linf : 0.0;
     spg_info info{};
     const int rc = h->s->solve(prm, &info);
     out.displacements.blocks.resize(nb);
@@ -368,13 +431,21 @@ void fem_solve(void* input, void* output) {
     out.error_code = rc;
     out.success = (rc == SPG_OK);
     out.info = "iterations=" + std::to_string(info.iterations) + " passes=" + std::to_string(info.passes) +
-               " r0_linf=" + std::to_string(info.r0_linf) + " r_linf=" + std::to_string(info.r_linf);
+               " r0_linf=" + std::to_string(info.r0_linf) + " r_linf=" + std::to_string(info.r_linf) +
+               (warm ? (info.warm_reused ? " warm_start=reused" : " warm_start=discarded") : "");
     if (rc != SPG_OK) out.info += " (not converged or failed)";
-    if (in.get_int("keep_state", 0)) {
-      keep = h;
-      out.set_state_ptr(h);
+    // what the next warm start begins from: the displacement of a finished
+    // solve, and the first solve's zero-guess residual
+    if (rc == SPG_OK || rc == SPG_E_NOT_CONVERGED) {
+      WarmState& w = h->warm;
+      w.base = base;
+      w.u.resize(nb * 192);
+      for (size_t i = 0; i < nb * 64; ++i)
+        for (int c = 0; c < 3; ++c) w.u[i * 3 + c] = packed[i * 4 + c];
+      w.initial_linf = prev.valid ? prev.initial_linf : info.r0_linf;
+      w.valid = true;
     } else {
-      delete h;
+      h->warm = std::move(prev);
     }
   } catch (SpgError& e) {
     out.success = false;
@@ -397,7 +468,8 @@ void fem_solve(void* input, void* output) {
     out.error_code = SPG_E_INVALID;
     out.info = "unknown error";
   }
-  (void)keep;
+  if (keep && h) out.set_state_ptr(h.release());
+  h.reset();
   try {
     emit(out, output);
   } catch (...) {

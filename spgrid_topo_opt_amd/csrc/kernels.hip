@@ -2679,7 +2679,7 @@ void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T*
   // us (one CU's VALU work).  The lists remain for sets beyond the int16
   // node tables and for a refused cooperative launch.
   if (path == 0 || path == 5) {
-    if (launch_bnd0_coop<T>(st, L, dx, u, b, sweeps, forward, path == 5)) return;
+    if (!L.coop_off && launch_bnd0_coop<T>(st, L, dx, u, b, sweeps, forward, path == 5)) return;
     path = 2;
   }
   if (path == 4 && launch_bnd0_phase<T>(st, L, dx, u, b, sweeps, forward)) return;

@@ -41,6 +41,7 @@ struct LevelDev {
   // either precision) and {barrier counter, error word}
   void* b0ub = nullptr;
   unsigned* coop = nullptr;
+  int coop_off = 0;  // 1 after a barrier time-out: the list launches take over
   // explicit levels: dense maps [G H] of one boundary phase (forward,
   // backward colour order) for `bmap_sweeps` sweeps, and their vector
   // scratch ([6n] input then [3n] output)

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <exception>
@@ -212,6 +213,16 @@ static uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
   return (spread(x) << 2) | (spread(y) << 1) | spread(z);
 }
 
+int64_t vector_index_limit() {
+  // SPG_INDEX_LIMIT lowers the limit so that tests can reach the check
+  // with small problems
+  if (const char* e = std::getenv("SPG_INDEX_LIMIT")) {
+    const long long v = std::atoll(e);
+    if (v > 0 && v < INT32_MAX) return v;
+  }
+  return INT32_MAX;
+}
+
 Layout build_layout(const ProblemDesc& p) {
   Layout out;
   if (p.nblocks <= 0) throw LayoutError("no density blocks", -1);
@@ -412,6 +423,12 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
     if (D.nb == 0) throw LayoutError("level " + std::to_string(l) + " has no active nodes", -1);
+    // the device kernels index level vectors ([nb][3][64]) with 32-bit ints
+    if (D.nb * 192 > vector_index_limit())
+      throw LayoutError("level " + std::to_string(l) + " holds " + std::to_string(D.nb) +
+                            " blocks: its vectors (192 entries per block) exceed the 32-bit index range of the "
+                            "device kernels (at most " + std::to_string(vector_index_limit() / 192) + " blocks)",
+                        -1);
     auto gfind = [&](int bx, int by, int bz) { return gmap[l].find(bx, by, bz); };
     D.nbr.resize(D.nb * 27);
     for (int64_t g = 0; g < D.nb; ++g) {

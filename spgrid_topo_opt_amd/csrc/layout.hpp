@@ -60,6 +60,9 @@ struct ProblemDesc {
   const double* dir_value;
 };
 
+// largest level vector (entries) the device kernels can index with int
+int64_t vector_index_limit();
+
 // SIMP map (topo_opt/fem_interface.h:343-353)
 double stiffness_mapping(double penalty, double min_stiffness, double min_density, double d);
 

@@ -29,9 +29,11 @@ constexpr int RT = 256;      // reduction workgroup size
 __device__ inline int side(int x) { return x < 0 ? 0 : (x > 3 ? 2 : 1); }
 
 // out(x) = sum_o w(o) v(x+o) / (sum_o w(o) m(x+o) (rho(x) + 1e-3)) with
-// v = dc rho [inside], m = [inside], w(o) = max(0, r - |o|), |o_i| <= h;
-// cells outside the container get 0.  The halo (4+2h)^3 of v and m is
-// staged in LDS per block.
+// v = dc rho [inside], m = [inside], w(o) = max(0, r - |o|), |o_i| <= h.
+// Cells outside the container take the same formula (smooth_dc writes every
+// cell of the fat blocks); where no cell of the window is inside, the
+// reference's 0/0 is written as 0.  The halo (4+2h)^3 of v and m is staged
+// in LDS per block.
 template <int H>
 __global__ void __launch_bounds__(64 * FW) k_filter(int64_t nb, const int* __restrict__ nbr,
                                                      const uint8_t* __restrict__ inside, const double* __restrict__ dc,
@@ -72,12 +74,17 @@ __global__ void __launch_bounds__(64 * FW) k_filter(int64_t nb, const int* __res
         num += wt * hv[w][h];
         den += wt * hm[w][h];
       }
-  out[c] = inside[c] ? num / (den * (rho[c] + 1e-3)) : 0.0;
+  out[c] = den > 0.0 ? num / (den * (rho[c] + 1e-3)) : 0.0;
 }
 
-// new density of one cell at multiplier mid (optimality_criterion.cpp:76-118)
-__device__ inline double oc_new(double rho, double dc, double mid, double step, double dmin) {
-  double v = rho * sqrt(fmax(0.0, -dc) / mid);
+// mask bits of the OC kernels
+constexpr uint8_t OC_INSIDE = 1;  // cell inside the container: updated
+constexpr uint8_t OC_FIXED = 2;   // fixed-density cell (flags.get_fixed_density())
+
+// new density of one cell at multiplier mid (optimality_criterion.cpp:84-101);
+// shift = min(0, min over the container of -dc) with smart_oc, else 0
+__device__ inline double oc_new(double rho, double dc, double mid, double step, double dmin, double shift) {
+  double v = rho * sqrt(fmax(0.0, -dc - shift) / mid);
   v = fmin(fmax(v, rho - step), rho + step);
   return fmin(fmax(v, dmin), 1.0);
 }
@@ -103,20 +110,28 @@ __device__ inline double block_max(double s) {
   return t;
 }
 
+struct OcParams {
+  double mid, step, dmin, fixed_density, shift;
+  int exclude_fixed;
+};
+
 // per-workgroup partial volume (and max change, with the new densities
 // written when out is given); the partials are added by k_oc_final in a
-// fixed order, so a trial is deterministic
+// fixed order, so a trial is deterministic.  Fixed cells take the fixed
+// density and leave the volume sum when exclude_fixed is set
+// (optimality_criterion.cpp:96-108).
 __global__ void __launch_bounds__(RT) k_oc_part(int64_t n, const uint8_t* __restrict__ mask,
-                                                const double* __restrict__ rho, const double* __restrict__ dc, double mid,
-                                                double step, double dmin, double* __restrict__ out,
-                                                double* __restrict__ part) {
+                                                const double* __restrict__ rho, const double* __restrict__ dc,
+                                                OcParams q, double* __restrict__ out, double* __restrict__ part) {
   double s = 0.0, ch = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
     const double r = rho[i];
+    const uint8_t m = mask[i];
     double v = r;
-    if (mask[i]) {
-      v = oc_new(r, dc[i], mid, step, dmin);
-      s += v;
+    if (m & OC_INSIDE) {
+      const bool fixed = (m & OC_FIXED) != 0;
+      v = fixed ? q.fixed_density : oc_new(r, dc[i], q.mid, q.step, q.dmin, q.shift);
+      if (!(fixed && q.exclude_fixed)) s += v;
       ch = fmax(ch, fabs(v - r));
     }
     if (out) out[i] = v;
@@ -181,9 +196,11 @@ extern "C" int spg_filter_sensitivity(int64_t nblocks, const int32_t* nbr, const
 }
 
 extern "C" int spg_oc_trial(int64_t n, const uint8_t* mask, const double* rho, const double* dc, double mid,
-                            double step, double min_density, double* out, double* volume, double* change) {
+                            double step, double min_density, double fixed_density, int32_t exclude_fixed,
+                            double dc_shift, double* out, double* volume, double* change) {
   if (!mask || !rho || !dc || !volume) return fail(SPG_E_INVALID, "null argument");
   if (!(mid > 0)) return fail(SPG_E_INVALID, "multiplier must be positive");
+  if (n < 0) return fail(SPG_E_INVALID, "negative cell count");
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail(SPG_E_HIP, "no HIP device");
   std::lock_guard<std::mutex> lock(g_oc_mu);
@@ -197,7 +214,8 @@ extern "C" int spg_oc_trial(int64_t n, const uint8_t* mask, const double* rho, c
       return fail(SPG_E_HIP, "OC scratch allocation failed");
   }
   const int g = (int)std::min<int64_t>(S.np, std::max<int64_t>(1, (n + RT - 1) / RT));
-  k_oc_part<<<g, RT>>>(n, mask, rho, dc, mid, step, min_density, out, S.part);
+  const OcParams q{mid, step, min_density, fixed_density, dc_shift, exclude_fixed ? 1 : 0};
+  k_oc_part<<<g, RT>>>(n, mask, rho, dc, q, out, S.part);
   k_oc_final<<<1, RT>>>(g, S.part, S.res);
   hipError_t e = hipMemcpy(S.host, S.res, sizeof(double) * 2, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail(SPG_E_HIP, std::string("OC trial: ") + hipGetErrorString(e));

@@ -34,6 +34,11 @@ static double now_s() {
 // boundary phases of explicit levels with at most this many unknowns run as
 // one dense map (2 maps x 3n x 6n entries in FP64 and FP32)
 static constexpr int kMaxDenseBoundary = 6144;
+// pcg() result: a level-0 boundary phase reported a barrier time-out; the
+// pass is repeated with the per-colour list launches
+static constexpr int kCoopRetry = -100;
+// byte offset of the barrier error word in the pinned scalar buffer
+static constexpr int kCoopWordOffset = 192;
 // relative residual reduction one FP32 MG-PCG pass is asked for at most
 static constexpr double kFp32PassReduction = 1e-6;
 // a bottom level above this many unknowns is refused (dense FP64 inverse)
@@ -72,6 +77,9 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   const double t0 = now_s();
   if (const char* e = std::getenv("SPG_BOTTOM_FP64")) bottom64_ = std::atoi(e) != 0;
   if (const char* e = std::getenv("SPG_COARSE_FP64")) coarse64_ = std::atoi(e) != 0;
+  // flags, hierarchy and the index-range check on the host, before any
+  // device call or allocation
+  lay_ = build_layout(pd);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw SpgError(SPG_E_HIP, "no HIP device available");
   if (cfg.device < 0 || cfg.device >= ndev) throw SpgError(SPG_E_INVALID, "device ordinal out of range");
@@ -81,7 +89,6 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
   RBCHK(rocblas_create_handle(&rb_));
   RBCHK(rocblas_set_stream(rb_, st_));
-  lay_ = build_layout(pd);
   double kmu[576], kla[576];
   element_tables(kmu, kla);
   for (int e = 0; e < 576; ++e) K0_[e] = cfg.mu * kmu[e] + cfg.lambda * kla[e];
@@ -359,6 +366,23 @@ void Solver::build_boundary_rows() {
   sync();
 }
 
+bool Solver::active_set_matches(const double* density) const {
+  const Layout& Lc = lay_;
+  for (int64_t b = 0; b < Lc.nin_blocks; ++b)
+    for (int l = 0; l < 64; ++l) {
+      const int32_t s = Lc.in_node_slot[b * 64 + l];
+      const double d = density[b * 64 + l];
+      if (std::isnan(d)) throw SpgError(SPG_E_NAN, "density contains NaN");
+      if (s < 0) {
+        // a cell whose min corner has no GPU slot is inactive; it may not turn on
+        if (d > 0) return false;
+        continue;
+      }
+      if ((Lc.E[s] != 0.0) != (d > 0)) return false;
+    }
+  return true;
+}
+
 void Solver::update_density(const double* density) {
   // recompute E (SIMP) for the existing cells, then the coarse operators
   const Layout& Lc = lay_;
@@ -589,9 +613,26 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
   launch_linf<T>(st_, r, n, part, &sc->nrm);
   int it = 0;
   int rc = SPG_E_NOT_CONVERGED;
+  unsigned* coop_word = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(hscal_) + kCoopWordOffset);
+  const bool coop = lv_[0].coop.p && !lv_[0].d.coop_off;
   for (;; ++it) {
     HIPCHK(hipMemcpyAsync(hscal_, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st_));
+    if (coop) HIPCHK(hipMemcpyAsync(coop_word, lv_[0].coop.as<unsigned>() + 1, sizeof(unsigned),
+                                    hipMemcpyDeviceToHost, st_));
     sync();
+    if (coop && *coop_word) {
+      // the last V-cycle's boundary phase ended early: its result is not a
+      // Gauss-Seidel sweep.  Switch to the list launches for good and let
+      // the caller repeat the pass.
+      HIPCHK(hipMemsetAsync(lv_[0].coop.as<unsigned>() + 1, 0, sizeof(unsigned), st_));
+      sync();
+      lv_[0].d.coop_off = 1;
+      std::fprintf(stderr,
+                   "spgrid_topo_opt_amd: level-0 boundary phase barrier timed out (workgroups not resident "
+                   "together); continuing with per-colour launches\n");
+      *iters = it;
+      return kCoopRetry;
+    }
     const CgScalars* h = reinterpret_cast<const CgScalars*>(hscal_);
     *last_nrm = h->nrm;
     if (print_residuals_) {
@@ -624,21 +665,49 @@ int Solver::pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm)
   return rc;
 }
 
-int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool) {
+void Solver::reset_to_ud(double* u) {
+  launch_zero<double>(st_, u, lv_[0].n3);
+  launch_set_entries(st_, u, ud_idx_.as<int>(), ud_val_.as<double>(), n_ud_);
+}
+
+int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool warm) {
   const LevelRT& R0 = lv_[0];
   const int64_t n = R0.n3;
   const double t0 = now_s();
   double* part = part_.as<double>();
   double* red = part + reduce_partials(0);
-  launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, u, f, res_.as<double>(), 1);
-  launch_linf<double>(st_, res_.as<double>(), n, part, red);
-  const double r0 = read_scalar(red);
+  auto residual_linf = [&](const double* x) {
+    launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, x, f, res_.as<double>(), 1);
+    launch_linf<double>(st_, res_.as<double>(), n, part, red);
+    return read_scalar(red);
+  };
+  // The target is relative to the zero-guess residual: this problem's own,
+  // or under a warm start the one stored from the first solve.  A warm
+  // guess is kept when forced or when its residual is below that reference
+  // (SPGrid_CPU_Solver.cpp:247-285, 318-326).
+  double r0 = 0.0, rstart = 0.0;
+  bool reused = false;
+  if (warm) {
+    r0 = prm.reference_linf;
+    if (!(r0 > 0)) {
+      reset_to_ud(tmp_.as<double>());
+      r0 = residual_linf(tmp_.as<double>());
+    }
+    rstart = residual_linf(u);
+    reused = prm.warm_start == 2 || r0 > rstart;
+    if (!reused) {
+      reset_to_ud(u);
+      rstart = residual_linf(u);
+    }
+  } else {
+    r0 = rstart = residual_linf(u);
+  }
   const double target = prm.tolerance * r0;
   print_residuals_ = prm.print_residuals;
   int total = 0, passes = 0, rc = SPG_OK;
-  double rn = r0;
-  bool conv = (r0 == 0.0);
-  if (std::isnan(r0)) rc = SPG_E_NAN;
+  double rn = rstart;
+  bool conv = (rstart <= target);
+  if (std::isnan(rstart) || std::isnan(r0)) rc = SPG_E_NAN;
   const int maxp = prm.max_passes < 1 ? 1 : prm.max_passes;
   for (int pass = 0; pass < maxp && !conv && rc == SPG_OK; ++pass) {
     // per-pass target: an explicit inner reduction, or (inner_tolerance <= 0)
@@ -650,15 +719,20 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
     int it = 0;
     double last = 0;
     int prc;
-    if (prm.precision == 1) {
-      launch_copy<double>(st_, res_.as<double>(), rr64_.as<double>(), n);
-      prc = pcg<double>(x64_.as<double>(), rr64_.as<double>(), tolp, prm.max_iterations, &it, &last);
-      launch_axpy_d<double>(st_, u, x64_.as<double>(), n);
-    } else {
-      launch_cast_d2f(st_, res_.as<double>(), rr32_.as<float>(), n);
-      prc = pcg<float>(x32_.as<float>(), rr32_.as<float>(), tolp, prm.max_iterations, &it, &last);
-      launch_axpy_d<float>(st_, u, x32_.as<float>(), n);
+    for (int attempt = 0;; ++attempt) {
+      if (prm.precision == 1) {
+        launch_copy<double>(st_, res_.as<double>(), rr64_.as<double>(), n);
+        prc = pcg<double>(x64_.as<double>(), rr64_.as<double>(), tolp, prm.max_iterations, &it, &last);
+      } else {
+        launch_cast_d2f(st_, res_.as<double>(), rr32_.as<float>(), n);
+        prc = pcg<float>(x32_.as<float>(), rr32_.as<float>(), tolp, prm.max_iterations, &it, &last);
+      }
+      if (prc != kCoopRetry) break;
+      total += it;
+      if (attempt > 0) throw SpgError(SPG_E_HIP, "boundary phase failed after the switch to list launches");
     }
+    if (prm.precision == 1) launch_axpy_d<double>(st_, u, x64_.as<double>(), n);
+    else launch_axpy_d<float>(st_, u, x32_.as<float>(), n);
     total += it;
     passes++;
     launch_apply<double>(st_, R0.d, E<double>(), cfg_.dx, u, f, res_.as<double>(), 1);
@@ -679,6 +753,8 @@ int Solver::solve_rhs(const spg_solve_params& prm, double* u, const double* f, s
     info->r_linf = rn;
     info->r_l2 = l2;
     info->solve_seconds = now_s() - t0;
+    info->r_start_linf = rstart;
+    info->warm_reused = reused ? 1 : 0;
   }
   check_coop();
   if (rc == SPG_OK && !conv) rc = SPG_E_NOT_CONVERGED;
@@ -698,16 +774,31 @@ void Solver::static_info(spg_info* info) const {
 int Solver::solve(const spg_solve_params& prm, spg_info* info) {
   const LevelRT& R0 = lv_[0];
   const int64_t n = R0.n3;
-  // u = u_D at the listed Dirichlet nodes (SPGrid_CPU_Solver.cpp:281-285)
-  launch_zero<double>(st_, u_.as<double>(), n);
-  launch_set_entries(st_, u_.as<double>(), ud_idx_.as<int>(), ud_val_.as<double>(), n_ud_);
+  if (prm.warm_start < 0 || prm.warm_start > 2) throw SpgError(SPG_E_INVALID, "warm_start must be 0, 1 or 2");
+  // a warm start begins at the previous solution (or the given guess) with
+  // u_D written over it; otherwise at u_D (SPGrid_CPU_Solver.cpp:241-285)
+  const bool warm = prm.warm_start != 0 && (have_solution_ || have_guess_);
+  if (warm) launch_set_entries(st_, u_.as<double>(), ud_idx_.as<int>(), ud_val_.as<double>(), n_ud_);
+  else reset_to_ud(u_.as<double>());
   // f = P P_null forces (SPGrid_CPU_Solver.cpp:289-299)
   launch_zero<double>(st_, f_.as<double>(), n);
   launch_set_entries(st_, f_.as<double>(), f_idx_.as<int>(), f_val_.as<double>(), n_f_);
   launch_project<double>(st_, R0.d, f_.as<double>(), true);
-  const int rc = solve_rhs(prm, u_.as<double>(), f_.as<double>(), info, true);
-  have_solution_ = true;
+  have_guess_ = false;
+  have_solution_ = false;
+  const int rc = solve_rhs(prm, u_.as<double>(), f_.as<double>(), info, warm);
+  // a failed solve leaves nothing to warm-start from
+  have_solution_ = rc == SPG_OK || rc == SPG_E_NOT_CONVERGED;
   return rc;
+}
+
+void Solver::set_initial_guess_in3(const double* dev) {
+  const LevelRT& R0 = lv_[0];
+  launch_zero<double>(st_, u_.as<double>(), R0.n3);
+  launch_scatter_input_vec(st_, in_slot_.as<int>(), lay_.nin_blocks * 64, dev, u_.as<double>());
+  launch_project<double>(st_, R0.d, u_.as<double>(), true);
+  sync();
+  have_guess_ = true;
 }
 
 int Solver::solve_adjoint(const spg_solve_params& prm, const double* g_in3, double* lam_in3, spg_info* info) {

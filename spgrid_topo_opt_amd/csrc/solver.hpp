@@ -67,6 +67,9 @@ class Solver {
   ~Solver();
 
   void set_forces(int64_t n, const int32_t* coord, const double* value);
+  // false if the densities would switch a cell on or off (a topology change
+  // that needs a new solver); throws on NaN
+  bool active_set_matches(const double* density) const;
   void update_density(const double* density);
   void update_density_device(const double* density_dev);  // input block order, device
   // returns SPG_OK or error code; fills info
@@ -74,6 +77,8 @@ class Solver {
   int solve_adjoint(const spg_solve_params& prm, const double* g_dev_in3, double* lam_dev_in3, spg_info* info);
   void pack_output(double* out4) const;  // host
   void displacement_in3(double* dev) const;
+  // starting guess of the next warm-started solve (input order, device)
+  void set_initial_guess_in3(const double* dev);
   double sensitivity(const double* u_in3_dev, const double* lam_in3_dev, double* dc_in_dev);
 
   // kernel-level hooks
@@ -101,7 +106,11 @@ class Solver {
   DBuf derr_;                // error flags of the device density check
   template <class T> const T* E() const;
   template <class T> T dxT() const { return (T)cfg_.dx; }
-  int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool with_ud);
+  // u holds the starting guess (u_D set); with `warm` it may be replaced by
+  // u_D when its residual is not below the reference (the reference's
+  // reuse test)
+  int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool warm);
+  void reset_to_ud(double* u);
   void build_operators();
   void build_boundary_rows();
   void upload_layout();
@@ -136,7 +145,8 @@ class Solver {
   double dev_bytes_ = 0.0;
   std::vector<std::vector<double>> posmat_cache_;
   std::unordered_map<uint64_t, int32_t> gmap0_;
-  bool have_solution_ = false;
+  bool have_solution_ = false;  // u_ holds a converged (or at least finite) solution
+  bool have_guess_ = false;     // u_ holds a guess from set_initial_guess_in3
 };
 
 }  // namespace spg

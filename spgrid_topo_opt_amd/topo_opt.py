@@ -34,12 +34,20 @@ class FEM:
     """A GPU solver bound to one block topology, with solve settings."""
 
     def __init__(self, prob, tolerance=1e-6, precision=0, inner_tolerance=0.0, max_passes=30, max_iterations=500,
-                 device=0):
+                 device=0, warm_start=0):
+        """warm_start: 0 every solve starts from u_D; 1 from the previous
+        solution when its residual is below the first solve's zero-guess
+        residual; 2 always from it (forced_reuse).  Under a warm start the
+        tolerance is relative to that first residual, as the reference's
+        kept state does (SPGrid_CPU_Solver.cpp:247-285, 318-326)."""
         self.prob = prob
         self.solver = _lib.Solver(prob, device=device)
         self.nin = prob.block_base.shape[0] * 64
         self.kw = dict(tolerance=tolerance, precision=precision, inner_tolerance=inner_tolerance if precision == 0
                        else 0.0, max_passes=max_passes if precision == 0 else 1, max_iterations=max_iterations)
+        self.warm_start = warm_start
+        self.reference_linf = 0.0  # zero-guess residual of the first solve
+        self.has_guess = False
         self.last_info = None
         self._rho_host = None
         self._rho_dev = None
@@ -78,9 +86,31 @@ class FEM:
         self.solver.close()
 
     def solve(self) -> dict:
-        _, info = self.solver.solve(want_output=False, **self.kw)
+        warm = self.warm_start if (self.reference_linf > 0 or self.has_guess) else 0
+        _, info = self.solver.solve(want_output=False, warm_start=warm, reference_linf=self.reference_linf,
+                                    **self.kw)
+        if self.warm_start and not self.reference_linf > 0:
+            self.reference_linf = info["r0_linf"]
+        self.has_guess = False
         self.last_info = info
         return info
+
+    def carry_over(self, old: "FEM", old_blocks: torch.Tensor, new_blocks: torch.Tensor, nblocks_total: int):
+        """Start the next warm solve from `old`'s solution: blocks are indices
+        into a common block list; blocks new to this solver start at zero."""
+        if not self.warm_start or old.last_info is None:
+            return
+        dev = new_blocks.device
+        u_old = old.displacement(dev).reshape(-1, 192)
+        pos = torch.full((nblocks_total,), -1, dtype=torch.int64, device=dev)
+        pos[old_blocks] = torch.arange(old_blocks.numel(), device=dev)
+        src = pos[new_blocks]
+        u_new = torch.where((src >= 0)[:, None], u_old[src.clamp(min=0)], torch.zeros_like(u_old[:1]))
+        u_new = u_new.reshape(-1).contiguous()
+        torch.cuda.synchronize(dev)
+        self.solver.set_initial_guess_device(u_new.data_ptr())
+        self.reference_linf = old.reference_linf
+        self.has_guess = True
 
     def displacement(self, device) -> torch.Tensor:
         u = torch.empty(self.nin * 3, dtype=torch.float64, device=device)
@@ -165,9 +195,11 @@ class CellGrid:
         self.inside_u8 = self.inside.to(torch.uint8).contiguous()
 
 def filter_sensitivity(grid: CellGrid, dc, rho, filter_size=1.5):
-    """dc1(x) = sum_o w(o) dc(x+o) rho(x+o) / (sum_o w(o) [x+o inside] (rho(x)+1e-3)),
+    """dc1(x) = sum_o w(o) dc(x+o) rho(x+o) [x+o inside] / (sum_o w(o) [x+o inside] (rho(x)+1e-3)),
     w(o) = max(0, r - |o|) over |o_i| <= floor(r) (spgrid_topo_opt.cpp:527-574),
-    by the HIP kernel spg_filter_sensitivity; 0 outside the container."""
+    by the HIP kernel spg_filter_sensitivity, for every cell (inside the
+    container or not, as smooth_dc writes every cell of the fat blocks; 0
+    where no window cell is inside)."""
     if not dc.is_cuda:
         raise RuntimeError("filter_sensitivity runs on the GPU (HIP kernel); got a CPU tensor")
     dc = dc.detach().to(torch.float64).contiguous()
@@ -179,33 +211,47 @@ def filter_sensitivity(grid: CellGrid, dc, rho, filter_size=1.5):
     return out
 
 
-def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None):
+def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None, fixed=None,
+              fixed_cell_density=1.0, exclude_fixed_cells=True, smart_oc=False):
     """Bisection on the Lagrange multiplier (optimizer/optimality_criterion.cpp:19-146):
-    new = clamp(old*sqrt(max(0,-dc)/mid), old-step, old+step), clamped to [min_density, 1],
-    on the `inside` cells; their volume must equal volume_fraction times the
-    number of container cells (default: the updated cells).  Each trial is the
-    HIP kernel spg_oc_trial; the bisection sequence is the reference's."""
+    new = clamp(old*sqrt(max(0,-dc-shift)/mid), old-step, old+step), clamped to
+    [min_density, 1], on the `inside` cells; `fixed` cells (a bool mask) take
+    fixed_cell_density instead and, with exclude_fixed_cells, neither count in
+    the volume nor in the container size.  shift = min(0, min over the inside
+    cells of -dc) with smart_oc, else 0.  The volume must equal volume_fraction
+    times (container cells - fixed inside cells); container cells default to
+    the updated cells.  Each trial is the HIP kernel spg_oc_trial; the
+    bisection sequence is the reference's."""
     if not rho.is_cuda:
         raise RuntimeError("oc_update runs on the GPU (HIP kernel); got a CPU tensor")
     n = inside.sum().item() if container_cells is None else container_cells
-    target = volume_fraction * n
+    mask = inside.to(torch.uint8)
+    n_fixed = 0
+    if fixed is not None:
+        fx = fixed & inside
+        mask = mask | (fx.to(torch.uint8) << 1)
+        if exclude_fixed_cells:
+            n_fixed = int(fx.sum().item())
+    target = volume_fraction * (n - n_fixed)
     rho = rho.detach().to(torch.float64).contiguous()
     dc = dc.detach().to(torch.float64).contiguous()
-    mask = inside.to(torch.uint8).contiguous()
+    mask = mask.contiguous()
+    shift = min(0.0, (-dc[inside]).min().item()) if smart_oc and bool(inside.any()) else 0.0
     torch.cuda.synchronize(rho.device)
+    kw = dict(fixed_density=fixed_cell_density, exclude_fixed=exclude_fixed_cells, dc_shift=shift)
     lower, upper = 1e-20, 1e20
     mid = 0.5 * (lower + upper)
     while lower * (1 + 1e-7) < upper:
         mid = 0.5 * (lower + upper)
         vol, _ = _lib.oc_trial(rho.numel(), mask.data_ptr(), rho.data_ptr(), dc.data_ptr(), mid, step_limit,
-                               min_density)
+                               min_density, **kw)
         if vol > target:
             lower = mid
         else:
             upper = mid
     new = torch.empty_like(rho)
     _, change = _lib.oc_trial(rho.numel(), mask.data_ptr(), rho.data_ptr(), dc.data_ptr(), mid, step_limit,
-                              min_density, new.data_ptr())
+                              min_density, new.data_ptr(), **kw)
     return new, change
 
 
@@ -223,8 +269,21 @@ class TopoOpt:
     (optimizer/optimality_criterion.cpp:29) do."""
 
     def __init__(self, prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, grid_update_start=None,
-                 active_threshold=1e-6, fat_margin=1, connectivity_filtering=True, **fem_kw):
+                 active_threshold=1e-6, fat_margin=1, connectivity_filtering=True, fixed=None,
+                 fixed_cell_density=1.0, exclude_fixed_cells=True, smart_oc=False, warm_starting=False,
+                 forced_reuse=True, **fem_kw):
+        """`fixed`: optional per-cell bool mask (input block order) of
+        fixed-density cells (flags.set_fixed_density, e.g. the cells that
+        add_force marks with fix_cells_near_force, spgrid_topo_opt.h:577-592);
+        they start at and keep fixed_cell_density.  exclude_fixed_cells and
+        smart_oc are the OC optimizer's options
+        (optimizer/optimality_criterion.cpp:31-66).  warm_starting and
+        forced_reuse are the caller's solver-state options
+        (spgrid_topo_opt.cpp:1013-1015): each solve starts from the previous
+        one's displacement, carried over when the band changes."""
         self.prob = prob
+        if warm_starting:
+            fem_kw = dict(fem_kw, warm_start=2 if forced_reuse else 1)
         self.fem_kw = fem_kw
         self.fem = FEM(prob, **fem_kw)
         self.grid = CellGrid(prob)
@@ -232,8 +291,13 @@ class TopoOpt:
         self.step_limit = step_limit
         self.filter_size = filter_size
         inside = self.grid.inside
+        self.fixed = None if fixed is None else torch.as_tensor(fixed, device=inside.device).reshape(-1) & inside
+        self.oc_kw = dict(fixed=self.fixed, fixed_cell_density=fixed_cell_density,
+                          exclude_fixed_cells=exclude_fixed_cells, smart_oc=smart_oc)
         self.rho = torch.where(inside, torch.full_like(inside, volume_fraction, dtype=torch.float64),
                                torch.zeros_like(inside, dtype=torch.float64))
+        if self.fixed is not None:
+            self.rho = torch.where(self.fixed, torch.full_like(self.rho, fixed_cell_density), self.rho)
         self.history = []
         self.iter = 0
         self.grid_update_start = grid_update_start
@@ -259,7 +323,8 @@ class TopoOpt:
         c = compliance(rho, self.fem)
         c.backward()
         dc = filter_sensitivity(self.grid, rho.grad.detach(), self.rho, self.filter_size)
-        new, change = oc_update(self.rho, dc, self.grid.inside, self.volume_fraction, self.step_limit)
+        new, change = oc_update(self.rho, dc, self.grid.inside, self.volume_fraction, self.step_limit,
+                                self.prob.min_density, **self.oc_kw)
         self.rho = new
         self.iter += 1
         rec = {"compliance": c.item(), "change": change, "iterations": self.fem.last_info["iterations"],
@@ -279,8 +344,10 @@ class TopoOpt:
             # new topology: a solver for the fatter blocks (fem_solve rebuilds its hierarchy likewise)
             sub = dataclasses.replace(self.prob, block_base=self.prob.block_base[blocks.cpu().numpy()],
                                       density=band.fem_density(self.rho, blocks).cpu().numpy())
-            self.fem.close()
+            old = self.fem
             self.fem = FEM(sub, **self.fem_kw)
+            self.fem.carry_over(old, self._fem_blocks, blocks, self.band.nb)
+            old.close()
             self._fem_blocks, self._fem_fat = blocks, band.fat.clone()
             rebuilt = True
         torch.cuda.synchronize(dev)
@@ -301,7 +368,7 @@ class TopoOpt:
         self.dc0 = torch.where(fat_cells, dc1, dc0)
         upd = self.grid.inside & fat_cells
         new, change = oc_update(self.rho, self.dc0, upd, self.volume_fraction, self.step_limit,
-                                self.prob.min_density, container_cells=self.grid.inside.sum().item())
+                                self.prob.min_density, container_cells=self.grid.inside.sum().item(), **self.oc_kw)
         self.rho = new
         torch.cuda.synchronize(dev)
         t3 = time.perf_counter()

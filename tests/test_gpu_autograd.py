@@ -235,8 +235,21 @@ def test_filter_and_oc_kernels_match_torch_reference(filter_size):
     want = TR.filter_sensitivity(grid, dc, rho, filter_size)
     assert torch.isfinite(got).all()
     assert ((got - want).abs().max() / want.abs().max()).item() < 1e-12
+    # cells outside the container with inside cells in their window carry a value too
+    assert (got[~grid.inside] != 0).any()
     new, change = T.oc_update(rho, got, grid.inside, 0.3, 0.2, 0.0)
     new_r, change_r, _ = TR.oc_update(rho, want, grid.inside, 0.3, 0.2, 0.0)
     assert (new - new_r).abs().max().item() < 1e-9
     assert abs(change - change_r) < 1e-9
     assert abs(new[grid.inside].sum().item() / grid.inside.sum().item() - 0.3) < 1e-6
+    # fixed-density cells (excluded and counted) and the smart_oc shift
+    fixed = torch.zeros_like(grid.inside)
+    fixed[torch.nonzero(grid.inside)[::9, 0]] = True
+    dc2 = got.clone()
+    dc2[torch.nonzero(grid.inside)[::5, 0]] = 0.3  # some cells gain nothing
+    for kw in (dict(fixed=fixed), dict(fixed=fixed, exclude_fixed_cells=False), dict(smart_oc=True),
+               dict(fixed=fixed, fixed_cell_density=0.8, smart_oc=True)):
+        new, change = T.oc_update(rho, dc2, grid.inside, 0.3, 0.2, 0.0, **kw)
+        new_r, change_r, _ = TR.oc_update(rho, dc2, grid.inside, 0.3, 0.2, 0.0, **kw)
+        assert (new - new_r).abs().max().item() < 1e-9, kw
+        assert abs(change - change_r) < 1e-9, kw

@@ -348,6 +348,25 @@ def test_cooperative_barrier_time_out_is_reported(wide_face, monkeypatch):
     assert rel(got, want) < 1e-12
 
 
+def test_solve_falls_back_to_list_launches_after_a_barrier_time_out(monkeypatch, capfd):
+    """a barrier wait cut to one tick during a solve: the pass is repeated
+    with the per-colour list launches (once per solver, with a message) and
+    the solve still reaches the oracle's solution"""
+    P = Pair(W.box(64, 32, 32, seed=3, mg_levels=3))
+    u_o, _, _ = P.o.solve(tolerance=1e-12, max_iterations=500)
+    monkeypatch.setenv("SPG_BOUNDARY_PATH", "5")
+    monkeypatch.setenv("SPG_COOP_LIMIT_TICKS", "1")
+    out, info = P.g.solve_mixed(tolerance=1e-8)
+    assert info["converged"] == 1
+    err = capfd.readouterr().err
+    assert err.count("continuing with per-colour launches") == 1, err
+    want = P.o.pack(u_o)[..., :3]
+    assert np.linalg.norm(out[..., :3] - want) / np.linalg.norm(want) < 1e-6
+    monkeypatch.delenv("SPG_COOP_LIMIT_TICKS")
+    out2, info2 = P.g.solve_mixed(tolerance=1e-8)  # stays on the list launches
+    assert info2["converged"] == 1 and "continuing" not in capfd.readouterr().err
+
+
 @pytest.mark.parametrize("levels", [0, 1, 3, 4])
 def test_fp64_solve_matches_oracle(levels):
     P = Pair(W.random_blocky(seed=21, n=(16, 12, 12), mg_levels=levels))

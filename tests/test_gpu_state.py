@@ -1,0 +1,138 @@
+"""GPU: the kept solver state of fem_solve (keep_state / forced_reuse,
+solver/SPGrid_CPU_Solver.cpp:58-75, 241-285, 318-326, 389-408).
+
+Each call goes through the C ABI with a serialized FEMInputs stream.  The
+state pointer of one call's FEMOutputs is passed into the next call's
+FEMInputs.  Every solution is checked against the FP64 oracle solve of the
+same inputs; warm starts must not change the answer, only the iterations.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from spgrid_topo_opt_amd import fem_interface as FI
+from spgrid_topo_opt_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _call(p, tol, state=None, keep=1, forced=1, fp64=0):
+    ei = {"keep_state": keep, "forced_reuse": forced, "defect_correction_iter": 20, "fp64_solve": fp64}
+    inp = FI.FEMInputs.from_problem(p, tolerance=tol, max_iterations=500, extra_ints=ei,
+                                    extra_doubles={"inner_tolerance": 1e-3})
+    if state:
+        inp.set_solver_state_ptr(state)
+    out = FI.FEMInterface(inp).run()
+    print(f"[fem_solve] {p.name} keep={keep} warm={state is not None} -> success={out.success} {out.info}",
+          flush=True)
+    return out, out.get_solver_state_ptr()
+
+
+def _iters(out):
+    return int(re.search(r"iterations=(\d+)", out.info).group(1))
+
+
+def _err(p, out):
+    o = O.Oracle(p)
+    u, _, info = o.solve(tolerance=1e-12, max_iterations=500)
+    assert info["rc"] == 0
+    want = o.pack(u)[..., :3]
+    return np.linalg.norm(out.displacements[..., :3] - want) / np.linalg.norm(want)
+
+
+def _with_density(p, density, name):
+    return W.Problem(**{**p.__dict__, "density": density, "name": name})
+
+
+@pytest.fixture(scope="module")
+def base():
+    return W.box(16, 8, 8, seed=1, mg_levels=3, load="point")
+
+
+def test_keep_state_density_change_active_set_change_and_release(base):
+    p1 = base
+    out1, s1 = _call(p1, 1e-10)
+    assert out1.success and s1, out1.info
+    assert _err(p1, out1) < 1e-8
+    # (i) new densities on the same cells: the kept hierarchy is updated
+    rng = np.random.default_rng(7)
+    d2 = np.where(p1.density > 0, 0.05 + 0.95 * rng.random(p1.density.shape), 0.0)
+    p2 = _with_density(p1, d2, "box16_density2")
+    out2, s2 = _call(p2, 1e-10, state=s1)
+    assert out2.success and s2, out2.info
+    assert "warm_start=" in out2.info
+    assert _err(p2, out2) < 1e-8
+    # (ii) a cell switched off on the same blocks: the solver is rebuilt
+    d3 = d2.copy()
+    b, l = np.argwhere(d3 > 0)[len(np.argwhere(d3 > 0)) // 2]
+    d3[b, l] = 0.0
+    p3 = _with_density(p2, d3, "box16_one_cell_off")
+    out3, s3 = _call(p3, 1e-10, state=s2)
+    assert out3.success and s3, out3.info
+    assert _err(p3, out3) < 1e-8
+    # ... and switched back on
+    out4, s4 = _call(p2, 1e-10, state=s3)
+    assert out4.success and s4, out4.info
+    assert _err(p2, out4) < 1e-8
+    # (iv) keep_state = 0 consumes the state: nothing is handed back
+    out5, s5 = _call(p2, 1e-10, state=s4, keep=0)
+    assert out5.success and s5 is None
+    assert _err(p2, out5) < 1e-8
+
+
+@pytest.mark.parametrize("forced", [1, 0])
+def test_warm_start_cuts_iterations(base, forced):
+    """A small density change: the solve started from the previous
+    displacement needs fewer PCG iterations than a cold one and reaches the
+    same solution.  The tolerance stays relative to the first solve's
+    zero-guess residual (SPGrid_CPU_Solver.cpp:318-326)."""
+    p1 = base
+    out1, s1 = _call(p1, 1e-8, fp64=1)
+    assert out1.success
+    rng = np.random.default_rng(11)
+    d2 = np.where(p1.density > 0, p1.density * (1 + 0.01 * rng.standard_normal(p1.density.shape)), 0.0)
+    p2 = _with_density(p1, d2, "box16_perturbed")
+    cold, _ = _call(p2, 1e-8, keep=0, fp64=1)
+    warm, s2 = _call(p2, 1e-8, state=s1, forced=forced, fp64=1)
+    assert cold.success and warm.success
+    assert "warm_start=reused" in warm.info
+    assert _iters(warm) < _iters(cold), (warm.info, cold.info)
+    assert _err(p2, warm) < 1e-6
+    _call(p2, 1e-8, state=s2, keep=0)  # release
+
+
+def test_warm_start_carries_u_across_a_block_change(base):
+    """A rebuilt solver (different block list) starts from the previous
+    displacement mapped by block coordinates."""
+    p1 = base
+    out1, s1 = _call(p1, 1e-8, fp64=1)
+    # a new block list: the same cells with slightly changed densities plus
+    # an empty (density 0) block away from the body
+    bb = np.concatenate([p1.block_base, np.array([[8, 8, 28]], np.int32)])
+    de = np.concatenate([p1.density * 1.01, np.zeros((1, 64))])
+    p2 = W.Problem(**{**p1.__dict__, "block_base": bb, "density": de, "name": "box16_extra_block",
+                      "res": (p1.res[0], p1.res[1], 48)})
+    cold, _ = _call(p2, 1e-8, keep=0, fp64=1)
+    warm, s2 = _call(p2, 1e-8, state=s1, fp64=1)
+    assert warm.success and "warm_start=reused" in warm.info
+    assert _iters(warm) < _iters(cold), (warm.info, cold.info)
+    assert _err(p2, warm) < 1e-6
+    _call(p2, 1e-8, state=s2, keep=0)
+
+
+def test_failed_call_releases_or_returns_the_state(base):
+    out1, s1 = _call(base, 1e-8)
+    assert out1.success and s1
+    bad = W.Problem(**{**base.__dict__, "mg_levels": 2})  # refused like the reference
+    out2, s2 = _call(bad, 1e-8, state=s1)
+    assert not out2.success and s2 is None  # the old state was freed, not leaked
+    # a failing solve on a kept, valid state hands it back
+    out3, s3 = _call(base, 1e-8)
+    p_nan = _with_density(base, np.where(base.density > 0, np.nan, 0.0), "box16_nan")
+    out4, s4 = _call(p_nan, 1e-8, state=s3)
+    assert not out4.success
+    if s4 is not None:
+        out5, s5 = _call(base, 1e-8, state=s4, keep=0)
+        assert out5.success and s5 is None

@@ -185,6 +185,21 @@ def test_fem_solve_without_device_fails_cleanly():
     assert "device" in out.info
 
 
+def test_index_range_is_checked_before_any_device_call(monkeypatch):
+    """Level vectors are indexed with 32-bit ints on the device: a level
+    beyond that range is refused on the host (SPG_E_INVALID), before the
+    device is touched.  SPG_INDEX_LIMIT lowers the limit to reach the check
+    with a small problem."""
+    from spgrid_topo_opt_amd import workloads as W
+    p = W.box(16, 8, 8)
+    monkeypatch.setenv("SPG_INDEX_LIMIT", str(192 * 10))
+    out = FI.FEMOutputs.deserialize(_lib.fem_solve_bytes(FI.FEMInputs.from_problem(p).serialize()))
+    assert not out.success and out.error_code == _lib.SPG_E_INVALID
+    assert "32-bit" in out.info and "at most 10 blocks" in out.info
+    with pytest.raises(_lib.SpgError, match="32-bit"):
+        _lib.Solver(p)
+
+
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "spgrid_topo_opt_amd.h")).read()
     import re

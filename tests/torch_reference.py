@@ -1,7 +1,8 @@
 """Plain-torch references of the optimizer-side HIP kernels (test use only):
 the sensitivity filter (topo_opt/spgrid_topo_opt.cpp:527-574) and the
 optimality-criterion bisection (topo_opt/optimizer/optimality_criterion.cpp:
-19-146), written with tensor ops over the same blocks."""
+19-146), written with tensor ops over the same blocks.  Pinned by closed-form
+cases in tests/test_torch_reference_pinning.py."""
 import numpy as np
 import torch
 
@@ -47,26 +48,34 @@ def filter_sensitivity(grid, dc, rho, filter_size=1.5):
                 sl = (slice(None), slice(h + a, h + a + 4), slice(h + b, h + b + 4), slice(h + c, h + c + 4))
                 num += w * hv[sl]
                 den += w * hm[sl]
-    out = num.reshape(-1) / (den.reshape(-1) * (rho + 1e-3))
-    return torch.where(grid.inside, out, torch.zeros_like(out))
+    den = den.reshape(-1)
+    out = num.reshape(-1) / (torch.where(den > 0, den, torch.ones_like(den)) * (rho + 1e-3))
+    return torch.where(den > 0, out, torch.zeros_like(out))
 
 
-def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None):
+def oc_update(rho, dc, inside, volume_fraction, step_limit=0.2, min_density=0.0, container_cells=None, fixed=None,
+              fixed_cell_density=1.0, exclude_fixed_cells=True, smart_oc=False):
     n = inside.sum().item() if container_cells is None else container_cells
+    fixed = torch.zeros_like(inside) if fixed is None else (fixed & inside)
+    if exclude_fixed_cells:
+        n -= fixed.sum().item()
     target = volume_fraction * n
-    neg = torch.clamp(-dc, min=0.0)
+    shift = min(0.0, (-dc[inside]).min().item()) if smart_oc else 0.0
+    neg = torch.clamp(-dc - shift, min=0.0)
+    counted = inside & ~fixed if exclude_fixed_cells else inside
 
     def trial(mid):
         new = torch.clamp(rho * torch.sqrt(neg / mid), rho - step_limit, rho + step_limit)
         new = torch.clamp(new, min_density, 1.0)
+        new = torch.where(fixed, torch.full_like(new, fixed_cell_density), new)
         return torch.where(inside, new, rho)
 
     lower, upper = 1e-20, 1e20
     while lower * (1 + 1e-7) < upper:
         mid = 0.5 * (lower + upper)
-        if trial(mid)[inside].sum().item() > target:
+        if trial(mid)[counted].sum().item() > target:
             lower = mid
         else:
             upper = mid
     new = trial(mid)
-    return new, (new - rho).abs().max().item(), mid
+    return new, (new - rho)[inside].abs().max().item(), mid
