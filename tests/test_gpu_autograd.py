@@ -235,8 +235,12 @@ def test_filter_and_oc_kernels_match_torch_reference(filter_size):
     want = TR.filter_sensitivity(grid, dc, rho, filter_size)
     assert torch.isfinite(got).all()
     assert ((got - want).abs().max() / want.abs().max()).item() < 1e-12
-    # cells outside the container with inside cells in their window carry a value too
-    assert (got[~grid.inside] != 0).any()
+    # cells outside the container with inside cells in their window carry a
+    # value too (for r = 1 the window is the cell alone, so they stay 0)
+    if filter_size > 1.0:
+        assert (got[~grid.inside] != 0).any()
+    else:
+        assert (got[~grid.inside] == 0).all()
     new, change = T.oc_update(rho, got, grid.inside, 0.3, 0.2, 0.0)
     new_r, change_r, _ = TR.oc_update(rho, want, grid.inside, 0.3, 0.2, 0.0)
     assert (new - new_r).abs().max().item() < 1e-9
