@@ -236,6 +236,25 @@ int spg_oc_trial(int64_t n, const uint8_t *mask, const double *rho, const double
                  double min_density, double fixed_density, int32_t exclude_fixed, double dc_shift, double *out,
                  double *volume, double *change);
 
+/* spg_oc_trial without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device or pinned memory (code under
+ * HIP graph capture, tight optimizer loops, harnesses that forbid hipMalloc).
+ * Same kernels and same result as spg_oc_trial, but
+ *   - the per-workgroup partials live in a fixed device array of
+ *     SPG_OC_NOALLOC_PARTS (volume, change) pairs (16 KiB), one copy per GPU,
+ *     held until the process ends; the grid is capped at that many
+ *     workgroups, which cover any n >= 0 by a grid-stride loop;
+ *   - result (device, double[2]) receives {volume, change};
+ *   - the kernels are enqueued on `stream` (a hipStream_t; NULL = the null
+ *     stream) and the call returns without synchronizing; launch errors are
+ *     reported, kernel faults surface at the caller's next synchronization.
+ * Calls that overlap on one GPU (other streams, other host threads) share
+ * that GPU's array and race: serialize them.                                 */
+#define SPG_OC_NOALLOC_PARTS 1024
+int spg_oc_trial_noalloc(int64_t n, const uint8_t *mask, const double *rho, const double *dc, double mid,
+                         double step, double min_density, double fixed_density, int32_t exclude_fixed,
+                         double dc_shift, double *out, double *result /*device [2]*/, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

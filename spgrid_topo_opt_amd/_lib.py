@@ -33,6 +33,7 @@ EXPORTS = [
     "spg_solve_adjoint_device", "spg_sensitivity_device", "spg_num_levels", "spg_level_size", "spg_level_nodes",
     "spg_kernel", "spg_level_stencils", "spg_bench_apply", "spg_bench_kernel", "spg_get_info",
     "spg_update_density_device", "spg_filter_sensitivity", "spg_oc_trial", "spg_set_initial_guess_device",
+    "spg_oc_trial_noalloc",
 ]
 
 
@@ -142,6 +143,8 @@ def lib():
         L.spg_filter_sensitivity.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_double, vp]
         L.spg_oc_trial.argtypes = [C.c_int64, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
                                    C.c_double, vp, dp, dp]
+        L.spg_oc_trial_noalloc.argtypes = [C.c_int64, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_int32, C.c_double, vp, vp, vp]
         L.fem_solve.argtypes = [vp, vp]
         L.fem_solve.restype = None
         L.fem_solve_test.argtypes = [vp, vp]
@@ -341,6 +344,18 @@ def oc_trial(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, out_ptr=None,
                               min_density, fixed_density, 1 if exclude_fixed else 0, dc_shift,
                               C.c_void_p(out_ptr) if out_ptr else None, C.byref(vol), C.byref(ch)))
     return vol.value, ch.value
+
+
+def oc_trial_noalloc(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, result_ptr, out_ptr=None,
+                     fixed_density=1.0, exclude_fixed=True, dc_shift=0.0, stream=0):
+    """spg_oc_trial_noalloc: one OC trial enqueued on `stream` (a raw
+    hipStream_t value, 0 = null stream) that allocates nothing; writes
+    {volume, change} to the device array at result_ptr and does not
+    synchronize"""
+    _check(lib().spg_oc_trial_noalloc(n, C.c_void_p(mask_ptr), C.c_void_p(rho_ptr), C.c_void_p(dc_ptr), mid, step,
+                                      min_density, fixed_density, 1 if exclude_fixed else 0, dc_shift,
+                                      C.c_void_p(out_ptr) if out_ptr else None, C.c_void_p(result_ptr),
+                                      C.c_void_p(stream) if stream else None))
 
 
 def fem_solve_bytes(data: bytes, test=False) -> bytes:

@@ -8,6 +8,7 @@
 // nbr[block*27 + s], s = (dx+1)*9 + (dy+1)*3 + (dz+1), -1 where absent.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -221,5 +222,55 @@ extern "C" int spg_oc_trial(int64_t n, const uint8_t* mask, const double* rho, c
   if (e != hipSuccess) return fail(SPG_E_HIP, std::string("OC trial: ") + hipGetErrorString(e));
   *volume = S.host[0];
   if (change) *change = S.host[1];
+  return SPG_OK;
+}
+
+// ------------------------------------------------ OC trial without allocation
+// Scratch of spg_oc_trial_noalloc: the per-workgroup (volume, change)
+// partials of k_oc_part, SPG_OC_NOALLOC_PARTS pairs (16 KiB).  Every GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is passed on.
+static __device__ __align__(256) double scratch[2 * SPG_OC_NOALLOC_PARTS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+double* g_na_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+extern "C" int spg_oc_trial_noalloc(int64_t n, const uint8_t* mask, const double* rho, const double* dc, double mid,
+                                    double step, double min_density, double fixed_density, int32_t exclude_fixed,
+                                    double dc_shift, double* out, double* result, void* stream) {
+  if (!mask || !rho || !dc || !result) return fail(SPG_E_INVALID, "null argument");
+  if (!(mid > 0)) return fail(SPG_E_INVALID, "multiplier must be positive");
+  if (n < 0) return fail(SPG_E_INVALID, "negative cell count");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(SPG_E_HIP, "no HIP device");
+  if (dev < 0 || dev >= kMaxDevices) return fail(SPG_E_INVALID, "device ordinal beyond the scratch table");
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(SPG_E_HIP, std::string("OC scratch symbol address: ") + hipGetErrorString(e));
+      g_na_ptr[dev] = static_cast<double*>(p);
+    }
+    part = g_na_ptr[dev];
+  }
+  // one partial pair per workgroup; the grid-stride loop of k_oc_part lets a
+  // grid of at most SPG_OC_NOALLOC_PARTS workgroups cover any n
+  const int64_t want = std::max<int64_t>(1, (n + RT - 1) / RT);
+  const int g = (int)std::min<int64_t>(SPG_OC_NOALLOC_PARTS, want);
+  if (g > SPG_OC_NOALLOC_PARTS) return fail(SPG_E_INVALID, "OC trial needs more partials than the scratch holds");
+  const OcParams q{mid, step, min_density, fixed_density, dc_shift, exclude_fixed ? 1 : 0};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // k_oc_part writes every partial k_oc_final reads: nothing to clear
+  k_oc_part<<<g, RT, 0, st>>>(n, mask, rho, dc, q, out, part);
+  k_oc_final<<<1, RT, 0, st>>>(g, part, result);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SPG_E_HIP, std::string("OC trial launch: ") + hipGetErrorString(e));
   return SPG_OK;
 }
