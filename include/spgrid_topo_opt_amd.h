@@ -106,7 +106,9 @@ typedef struct {
   double hbm_bytes;        /* device memory held by the solver              */
   double r_start_linf;     /* FP64 residual (L_inf) of the starting guess   */
   int32_t warm_reused;     /* 1: the solve started from the previous u      */
-  int32_t reserved1;
+  int32_t bottom_kind;     /* 0: dense FP64 inverse of the bottom matrix;
+                              1: nested-dissection sparse FP64 Cholesky      */
+  double bottom_bytes;     /* device bytes of the bottom solve's factors     */
 } spg_info;
 
 const char *spg_last_error(void);

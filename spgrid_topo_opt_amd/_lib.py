@@ -91,7 +91,8 @@ class Info(C.Structure):
         ("hbm_bytes", C.c_double),
         ("r_start_linf", C.c_double),
         ("warm_reused", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("bottom_kind", C.c_int32),
+        ("bottom_bytes", C.c_double),
     ]
 
     def as_dict(self):

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libspgrid_topo_opt_amd.so")
-SOURCES = ["layout.cpp", "kernels.hip", "solver.hip", "capi.hip", "opt_kernels.hip"]
+SOURCES = ["layout.cpp", "kernels.hip", "solver.hip", "capi.hip", "opt_kernels.hip", "bottom_nd.hip"]
 ARCH = os.environ.get("SPG_OFFLOAD_ARCH", "gfx950")
 
 

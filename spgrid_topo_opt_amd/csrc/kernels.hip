@@ -2648,7 +2648,11 @@ template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
   const char* e = std::getenv("SPG_GS_KERNEL");
   // measured on gfx950 (256^3 shell, per 8-colour sweep): quad 374 / 535 us,
-  // octet 455 / 580 us, corner-per-wave 533 / 898 us (FP32 / FP64)
+  // octet 455 / 580 us, corner-per-wave 533 / 898 us (FP32 / FP64); two
+  // more layouts were measured against the quad's later 302 / 445 us and
+  // dropped: a lane per node over 8 blocks per wavefront (456 / 737 us,
+  // 3 waves per SIMD leave the halo loads exposed) and two lanes per node
+  // with the x-mirror split (315-352 / 541-653 us)
   const int kind = e && std::atoi(e) > 0 ? std::atoi(e) : 3;
   if (kind == 2) {
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);

@@ -41,8 +41,12 @@ static constexpr int kCoopRetry = -100;
 static constexpr int kCoopWordOffset = 192;
 // relative residual reduction one FP32 MG-PCG pass is asked for at most
 static constexpr double kFp32PassReduction = 1e-6;
-// a bottom level above this many unknowns is refused (dense FP64 inverse)
-static constexpr int64_t kMaxBottomDofs = 60000;
+// bottom solve: the dense FP64 inverse up to this many unknowns (one
+// bandwidth-bound GEMV per V-cycle), the nested-dissection Cholesky
+// (bottom_nd.hip) above it; SPG_BOTTOM=dense|nd overrides (dense stays
+// limited to kMaxDenseBottomDofs: its inverse is O(n^2))
+static constexpr int64_t kDenseBottomDofs = 16384;
+static constexpr int64_t kMaxDenseBottomDofs = 60000;
 
 DBuf::~DBuf() {
   if (p) (void)hipFree(p);
@@ -262,9 +266,31 @@ void Solver::build_operators() {
       nv += 3;
     }
   nvar_ = nv;
-  if (nv > kMaxBottomDofs)
-    throw SpgError(SPG_E_INVALID, "bottom level has " + std::to_string(nv) + " unknowns (limit " +
-                                      std::to_string(kMaxBottomDofs) + "); increase mg_level");
+  const char* bsel = std::getenv("SPG_BOTTOM");
+  const bool want_nd = bsel ? std::string(bsel) == "nd" : nv > kDenseBottomDofs;
+  if (!want_nd && nv > kMaxDenseBottomDofs)
+    throw SpgError(SPG_E_INVALID, "bottom level has " + std::to_string(nv) + " unknowns, more than the dense "
+                                      "inverse takes (" + std::to_string(kMaxDenseBottomDofs) + "); SPG_BOTTOM=dense "
+                                      "asked for it");
+  if (want_nd) {
+    // nested-dissection Cholesky: symbolic once per topology, numeric for
+    // every new set of stencils
+    try {
+      if (!nd_) {
+        nd_ = std::make_unique<NdBottom>(D.nb, D.base.data(), D.nbr.data(), D.flags.data(), D_ACTIVE);
+        nd_->upload(st_);
+      }
+      nd_->factor(lv_[B].st64.as<double>(), st_, rb_);
+    } catch (const NdNotSpd& e) {
+      throw SpgError(SPG_E_SINGULAR, e.what());
+    } catch (const SpgError&) {
+      throw;
+    } catch (const std::exception& e) {
+      throw SpgError(SPG_E_HIP, std::string("nested-dissection bottom: ") + e.what());
+    }
+    return;
+  }
+  nd_.reset();
   upload(var_of_, var);
   DBuf Ainv;
   Ainv.alloc((size_t)nv * nv * 8);
@@ -488,6 +514,13 @@ void Solver::level_residual(int l, const T* u, const T* b, T* r) {
 template <class T>
 void Solver::bottom(const T* b, T* u) {
   const LevelRT& R = lv_[lay_.nlev - 1];
+  if (nd_) {
+    // FP64 factors in both cycles: the FP32 cycle's right-hand side is
+    // widened and the solution rounded, as the reference casts the PARDISO
+    // result to float (Multigrid_Hierarchy.h:346-372, 439-451)
+    nd_->solve<T>(R.d.nb, b, u, st_);
+    return;
+  }
   const bool f = sizeof(T) == 4;
   T* x = f ? (T*)xb32_.p : (T*)xb64_.p;
   T* y = f ? (T*)yb32_.p : (T*)yb64_.p;
@@ -769,6 +802,9 @@ void Solver::static_info(spg_info* info) const {
   info->gpu_blocks = lay_.lv[0].nb;
   info->bottom_dofs = nvar_;
   info->hbm_bytes = dev_bytes_;
+  info->bottom_kind = nd_ ? 1 : 0;
+  info->bottom_bytes = nd_ ? (double)nd_->device_bytes()
+                           : (double)(tiles64_.bytes + tiles32_.bytes + pb64_.bytes + pb32_.bytes);
 }
 
 int Solver::solve(const spg_solve_params& prm, spg_info* info) {

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/spgrid_topo_opt_amd.h"
+#include "bottom_nd.hpp"
 #include "kernels.hpp"
 #include "layout.hpp"
 
@@ -135,6 +136,7 @@ class Solver {
   bool bottom64_ = false;  // FP32 cycles apply the FP64 bottom inverse
   bool coarse64_ = false;  // FP32 cycles run the explicit levels in FP64
   DBuf var_of_, info_, tiles64_, tiles32_, xb64_, yb64_, xb32_, yb32_, pb64_, pb32_;
+  std::unique_ptr<NdBottom> nd_;  // nested-dissection Cholesky bottom (large bottoms)
   // PCG / outer vectors (level 0)
   DBuf x32_, rr32_, s32_, q32_, z32_, x64_, rr64_, s64_, q64_, z64_;
   DBuf u_, f_, res_, tmp_;

@@ -381,10 +381,11 @@ def test_fp64_solve_matches_oracle(levels):
     assert ig["r_linf"] <= tol * ig["r0_linf"]
 
 
-def test_bottom_with_more_than_2_31_inverse_entries():
+def test_bottom_with_more_than_2_31_inverse_entries(monkeypatch):
     """a bottom level of more than 46341 unknowns (its dense inverse has more
     than 2^31 entries and takes the 64-bit factorization path) still solves
     exactly: FP64 MG-PCG iterations and solution agree with the oracle"""
+    monkeypatch.setenv("SPG_BOTTOM", "dense")  # bottoms this large take the sparse solve by default
     P = Pair(W.ellipsoid_shell(256, 128, 128, thickness_cells=2, mg_levels=3))
     nv = P.g.info()["bottom_dofs"]
     print("bottom unknowns", nv, flush=True)
