@@ -7,7 +7,13 @@ defect-correction loop) of K(rho) u = f from a zero initial guess to
 JSON line on stdout; logs go to stderr.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--levels 5]
-                  [--no-cpu] [--cpu-seconds 20]
+                  [--no-cpu] [--cpu-runs 5]
+
+Beside the timed GPU solves it reports the accuracy of both GPU solves
+against the FP64 oracle's solve to 1e-10 and, as the CPU baseline, the
+oracle's own solve time to the same tolerance on the host's cores (median of
+--cpu-runs); --no-cpu skips the oracle and compares with the GPU's own FP64
+solve instead.
 
 With --opt-loop it instead times the SIMP/OC optimization loop on the
 512x256x256 band (cells within 24 cells of the bounding-box surface):
@@ -29,6 +35,12 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE_FP32 = 28.0  # u 12 + f 12 + E 4 (algorithmic, per active node)
 BYTES_PER_NODE_FP64 = 56.0
 GS_SWEEP_BYTES_PER_NODE_FP32 = 40.0  # one 8-colour sweep: u read + write 24, b 12, E 4 (algorithmic)
+GS_KERNEL = "k_gs0_colour4<float, 8>"
+APPLY_KERNEL = "k_apply_lds<float, double, 2>"
+# what the mixed solve stores and sums in (the FP64 row sums of the Krylov
+# operator depart from an all-FP32 inner solve; they keep its iteration count)
+ACCUMULATION = ("FP32 storage and FP32 sums in smoothing, transfers and V-cycle residuals; the PCG operator "
+                "q = P K s reads and writes FP32 but sums its rows in FP64; FP64 outer residual, dots and bottom")
 
 
 def log(*a):
@@ -46,7 +58,7 @@ def main():
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed oracle solves (median reported)")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--max-iterations", type=int, default=500, help="PCG iterations per pass (FP64 solve: 4x)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -57,6 +69,9 @@ def main():
     ap.add_argument("--opt-band-start", type=int, default=5)
     ap.add_argument("--opt-tol", type=float, default=1e-4,
                     help="solve tolerance in the loop (the reference optimizer's cg_tolerance default)")
+    ap.add_argument("--opt-warm", action="store_true",
+                    help="warm-start every solve from the previous displacement (the caller's warm_starting with "
+                         "forced_reuse; off by default, as in the reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -142,15 +157,18 @@ def main():
     log(f"[bench] fp64 solve {t_fp64 * 1e3:.2f} ms, iterations {i64['iterations']}, "
         f"rel residual {i64['r_linf'] / i64['r0_linf']:.2e}")
 
-    # accuracy against a tight FP64 reference solve (10^-10) on the same
-    # problem: relative L2 displacement error and relative compliance error
-    acc = accuracy_report(s, args, pr)
-    log(f"[bench] vs FP64 reference: mixed u err {acc['mixed']['u_rel_l2']:.2e}, c err {acc['mixed']['c_rel']:.2e}; "
-        f"fp64 u err {acc['fp64']['u_rel_l2']:.2e}, c err {acc['fp64']['c_rel']:.2e}")
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s (read + write)")
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(prob, info["iterations"], args.cpu_seconds)
+    # the CPU leg: the FP64 oracle (the reference's algorithm restated in C,
+    # OpenMP) solves the same problem to 1e-10 (the accuracy reference, and
+    # the warm-up), then to args.tol five times (the CPU baseline)
+    cpu, ref = (None, None) if args.no_cpu else cpu_leg(prob, args.tol, args.cpu_runs)
+    # accuracy of the GPU mixed and FP64 solves (to args.tol) against that
+    # reference: relative L2 displacement error and relative compliance error
+    acc = accuracy_report(s, args, pr, ref)
+    log(f"[bench] vs {acc['reference']['source']}: mixed u err {acc['mixed']['u_rel_l2']:.2e}, "
+        f"c err {acc['mixed']['c_rel']:.2e}; fp64 u err {acc['fp64']['u_rel_l2']:.2e}, c err {acc['fp64']['c_rel']:.2e}")
 
     out = {
         "metric": METRIC,
@@ -164,6 +182,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32-inner/fp64-outer",
+        "accumulation": ACCUMULATION,
         "data": "synthetic",
         "config": {
             "workload": workload,
@@ -193,60 +212,160 @@ def main():
         "apply_us_fp64": t_ap64 * 1e6,
         "vcycle_ms_fp32": t_vc32 * 1e3,
         "vcycle_ms_fp64": t_vc64 * 1e3,
+        # the kernel with the largest share of the timed step: the level-0
+        # colour Gauss-Seidel launches (8 per sweep, 2 sweeps per V-cycle)
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_apply_lds<float, double, 2> (level-0 stiffness apply of the PCG operator: FP32 in/out, "
-                      "FP64 row sums)",
+            "kernel": GS_KERNEL + " (level-0 Gauss-Seidel colour launch; 8 per sweep)",
+            "achieved": GS_SWEEP_BYTES_PER_NODE_FP32 / 8 * nn / (t_gs32 / 8) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic(GS_KERNEL, nn),
+            "us_per_launch": t_gs32 / 8 * 1e6,
+            "algorithmic_bytes_per_launch": GS_SWEEP_BYTES_PER_NODE_FP32 / 8 * nn,
+        },
+        "roofline_apply": {
+            "bound": "hbm",
+            "kernel": APPLY_KERNEL + " (level-0 stiffness apply of the PCG operator)",
             "achieved": gbs32,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
-            "traffic": measured_traffic("k_apply_lds<float, double, 2>", nn),
+            "traffic": measured_traffic(APPLY_KERNEL, nn),
+            "us_per_launch": t_ap32 * 1e6,
+            "algorithmic_bytes_per_launch": BYTES_PER_NODE_FP32 * nn,
         },
-        "roofline_gs_sweep": {
-            "bound": "hbm",
-            "kernel": "k_gs0_colour4<float, 8> (level-0 Gauss-Seidel, 8 colour launches per sweep)",
-            "achieved": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9 / HBM_PEAK_GBS,
-            "us_per_sweep": t_gs32 * 1e6,
-            "traffic_per_sweep": (lambda t: None if t is None else 8 * t)(
-                measured_traffic("k_gs0_colour4<float, 8>", nn)),
-        },
+        "copy_bandwidth_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_reference": acc,
     }
     print(json.dumps(out), flush=True)
 
 
-def accuracy_report(s, args, pr):
+def accuracy_report(s, args, pr, ref):
     """Relative L2 displacement error and relative compliance error of the
-    mixed and the FP64 solve (both to args.tol) against an FP64 solve to
-    10^-10; the compliance is the sensitivity kernel's objective
-    (sum_e u_e^T K0 u_e E(rho_e))."""
+    GPU mixed and FP64 solves (both to args.tol) against `ref`, the FP64
+    oracle's solution to 10^-10 (displacements in the output block order and
+    the oracle's compliance sum_e u_e^T K0 u_e E(rho_e)).  Without the CPU
+    leg the reference is the GPU's own FP64 solve to 10^-10."""
+    import numpy as np
     import torch
 
-    n = s.nblocks * 64 * 3
     dc = torch.empty(s.nblocks * 64, dtype=torch.float64, device="cuda")
 
     def run(**kw):
-        _, info = s.solve(want_output=False, raise_on_error=False, print_residuals=pr, **kw)
-        u = torch.empty(n, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        s.displacement_device(u.data_ptr())
-        c = s.sensitivity_device(dc.data_ptr(), u_ptr=u.data_ptr())
-        return u, c, info
+        out, info = s.solve(want_output=True, raise_on_error=False, print_residuals=pr, **kw)
+        c = s.sensitivity_device(dc.data_ptr())
+        return out[..., :3].copy(), c, info
 
-    u_ref, c_ref, i_ref = run(tolerance=1e-10, precision=1, max_passes=1, max_iterations=8 * args.max_iterations)
-    out = {"reference": {"tolerance": 1e-10, "iterations": i_ref["iterations"], "converged": bool(i_ref["converged"]),
-                         "compliance": c_ref}}
+    if ref is None:
+        u_ref, c_ref, i_ref = run(tolerance=1e-10, precision=1, max_passes=1, max_iterations=8 * args.max_iterations)
+        head = {"source": "GPU FP64 solve (no CPU leg)", "tolerance": 1e-10, "iterations": i_ref["iterations"],
+                "converged": bool(i_ref["converged"]), "compliance": c_ref}
+    else:
+        u_ref, c_ref = ref["u"], ref["compliance"]
+        head = {"source": "FP64 oracle solve", "tolerance": ref["tolerance"], "iterations": ref["iterations"],
+                "converged": ref["converged"], "compliance": c_ref}
+    out = {"reference": head}
     for name, kw in (("mixed", dict(precision=0, max_passes=30, max_iterations=args.max_iterations)),
                      ("fp64", dict(precision=1, max_passes=1, max_iterations=4 * args.max_iterations))):
         u, c, info = run(tolerance=args.tol, **kw)
-        out[name] = {"u_rel_l2": float(torch.linalg.vector_norm(u - u_ref) / torch.linalg.vector_norm(u_ref)),
-                     "c_rel": abs(c - c_ref) / abs(c_ref), "iterations": info["iterations"]}
+        out[name] = {"u_rel_l2": float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)),
+                     "c_rel": abs(c - c_ref) / abs(c_ref), "iterations": info["iterations"],
+                     "rel_residual": info["r_linf"] / info["r0_linf"]}
     return out
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Read + write GB/s of a plain device-to-device copy (torch)."""
+    import torch
+
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    b.synchronize()
+    t = a.elapsed_time(b) / 1e3 / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * nbytes / t / 1e9
+
+
+def host_cpu():
+    """CPU model, logical and physical core counts of this host."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if cur:
+                        phys.add((cur.get("physical id"), cur.get("core id")))
+                        cur = {}
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(phys) or None,
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_leg(prob, tol, runs):
+    """The FP64 oracle (C, OpenMP) on the same problem: one solve to 1e-10
+    (warm-up, and the accuracy reference), then `runs` timed solves to `tol`
+    from zero; setup (flags, hierarchy, Galerkin, bottom factorization) and
+    solve are reported separately, the solve as the median."""
+    import statistics
+
+    os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the OpenMP runtime starts
+    from oracle import oracle as O
+
+    host = host_cpu()  # before the OpenMP runtime binds this thread to one core
+    saved = os.sched_getaffinity(0)
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(saved)))
+    t0 = time.perf_counter()
+    o = O.Oracle(prob)
+    setup = time.perf_counter() - t0
+    u, _, info = o.solve(tolerance=1e-10, max_iterations=2000)
+    c_ref, _, _ = o.sensitivity(u)
+    ref = {"u": o.pack(u)[..., :3], "compliance": c_ref, "tolerance": 1e-10, "iterations": info["iterations"],
+           "converged": bool(info["converged"])}
+    log(f"[bench] cpu oracle: setup {setup:.1f}s, reference solve to 1e-10 in {info['solve_seconds']:.1f}s "
+        f"({info['iterations']} iterations)")
+    secs, its = [], []
+    for _ in range(runs):
+        _, _, inf = o.solve(tolerance=tol, max_iterations=2000)
+        secs.append(inf["solve_seconds"])
+        its.append(inf["iterations"])
+    os.sched_setaffinity(0, saved)  # the bound master thread returns to all cores
+    med = statistics.median(secs)
+    log(f"[bench] cpu oracle: solve to {tol:g} median {med:.2f}s of {runs} ({its[0]} iterations) on {cores} threads")
+    del o
+    cpu = {
+        "value": med,
+        "unit": "s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"the same problem solved by the FP64 oracle to {tol:g} relative L-inf residual from zero, "
+                  f"median of {runs} solves after one warm-up solve (to 1e-10), setup excluded",
+        "setup_s": setup,
+        "solve_s_runs": secs,
+        "iterations": its[0],
+        "host": host,
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+        "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+    }
+    return cpu, ref
 
 
 COPY_BYTES = 512 << 20
@@ -292,33 +411,6 @@ def traffic_probe(s, prob):
     log(f"[bench] traffic probe done: FP32 apply x5+1, GS sweep, boundary phases, copy of {COPY_BYTES} B x3")
 
 
-def cpu_baseline(prob, gpu_iterations, budget_s):
-    """Oracle (C, OpenMP, FP64) on the same problem: time PCG iterations for
-    about `budget_s` seconds and scale to the GPU's iteration count."""
-    from oracle import oracle as O
-
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    o = O.Oracle(prob)
-    setup = time.perf_counter() - t0
-    _, sec1 = o.time_pcg_iterations(1)
-    k = max(1, min(50, int(budget_s / max(sec1, 1e-3))))
-    it, sec = o.time_pcg_iterations(k)
-    per_it = sec / max(it, 1)
-    log(f"[bench] cpu oracle: setup {setup:.1f}s, {per_it:.3f}s per PCG iteration ({k} timed) on {cores} threads")
-    return {
-        "value": per_it * gpu_iterations,
-        "unit": "s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"{k} FP64 MG-PCG iterations of the same shell problem timed on the host; value = time per "
-                  f"iteration x {gpu_iterations} iterations (the GPU mixed solve's count); oracle setup "
-                  f"{setup:.1f}s not included",
-        "setup_s": setup,
-        "per_iteration_s": per_it,
-    }
-
-
 def opt_loop(args):
     """SIMP/OC loop (volume fraction 0.1, step 0.2, filter 1.5) on the band
     workload, narrow band updated every iteration from iteration 5."""
@@ -336,7 +428,7 @@ def opt_loop(args):
         prec = {"mixed": 0, "fp64": 1}[name]
         t0 = time.perf_counter()
         opt = T.TopoOpt(prob, volume_fraction=0.1, step_limit=0.2, filter_size=1.5, grid_update_start=args.opt_band_start,
-                        tolerance=args.opt_tol, precision=prec)
+                        tolerance=args.opt_tol, precision=prec, warm_starting=args.opt_warm)
         torch.cuda.synchronize()
         setup = time.perf_counter() - t0
         log(f"[opt] {name}: setup {setup:.2f}s")
@@ -375,7 +467,7 @@ def opt_loop(args):
         "data": "synthetic",
         "config": {"workload": "band 512x256x256, cells within 24 cells of the bounding-box surface, "
                                "V=0.1, step 0.2, filter 1.5, 6 MG levels",
-                   "band_update_from_iteration": args.opt_band_start,
+                   "band_update_from_iteration": args.opt_band_start, "warm_start": bool(args.opt_warm),
                    "active_cells": prob.n_active_cells, "tolerance": args.opt_tol},
         "runs": runs,
     }
