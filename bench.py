@@ -36,11 +36,11 @@ BYTES_PER_NODE_FP32 = 28.0  # u 12 + f 12 + E 4 (algorithmic, per active node)
 BYTES_PER_NODE_FP64 = 56.0
 GS_SWEEP_BYTES_PER_NODE_FP32 = 40.0  # one 8-colour sweep: u read + write 24, b 12, E 4 (algorithmic)
 GS_KERNEL = "k_gs0_colour4<float, 8>"
-APPLY_KERNEL = "k_apply_lds<float, double, 2>"
-# what the mixed solve stores and sums in (the FP64 row sums of the Krylov
-# operator depart from an all-FP32 inner solve; they keep its iteration count)
-ACCUMULATION = ("FP32 storage and FP32 sums in smoothing, transfers and V-cycle residuals; the PCG operator "
-                "q = P K s reads and writes FP32 but sums its rows in FP64; FP64 outer residual, dots and bottom")
+APPLY_KERNEL = "k_apply_cell<float, 2>"
+# what the mixed solve stores and sums in
+ACCUMULATION = ("FP32 storage and FP32 sums throughout the inner MG-PCG (the Krylov operator q = P K s cell by cell "
+                "in strain form, where rigid motions cancel exactly); FP64 outer residual, PCG scalars and dots, "
+                "Galerkin assembly and bottom solve")
 
 
 def log(*a):
@@ -107,16 +107,18 @@ def main():
 
     # kernel throughput first: it does not depend on how the solve converges
     nn = s.info()["active_nodes"]
-    t_ap32 = s.bench_apply(0, args.apply_reps)  # the PCG operator: FP32 in/out, FP64 row sums
+    t_ap32 = s.bench_apply(0, args.apply_reps)  # the PCG operator: FP32, cell by cell in strain form
     t_ap64 = s.bench_apply(1, args.apply_reps)
-    # the same apply in difference form with FP32 sums, then with direct FP32 sums
+    # the same operator as node rows: FP64 row sums, difference form, direct FP32 sums
+    os.environ["SPG_APPLY_CELL"] = "0"
+    t_ap32r = s.bench_apply(0, args.apply_reps)
     os.environ["SPG_DIFF_SUMS"] = "1"
     t_ap32d = s.bench_apply(0, args.apply_reps)
     os.environ["SPG_DIFF_SUMS"] = "0"
     os.environ["SPG_ACC64_MODES"] = "0"
     t_ap32f = s.bench_apply(0, args.apply_reps)
-    del os.environ["SPG_ACC64_MODES"]
-    del os.environ["SPG_DIFF_SUMS"]
+    for k in ("SPG_ACC64_MODES", "SPG_DIFF_SUMS", "SPG_APPLY_CELL"):
+        del os.environ[k]
     t_gs32 = s.bench_kernel(_lib.OP_GS_SWEEP, 0, 0, 0, 20)  # 8 colour launches
     t_vc32 = s.bench_kernel(_lib.OP_VCYCLE, 0, 0, 0, 5)
     t_vc64 = s.bench_kernel(_lib.OP_VCYCLE, 0, 1, 0, 5)
@@ -206,6 +208,7 @@ def main():
         "apply_mdof_s_fp32": 3 * nn / t_ap32 / 1e6,
         "apply_mdof_s_fp64": 3 * nn / t_ap64 / 1e6,
         "apply_us_fp32": t_ap32 * 1e6,
+        "apply_us_fp32_rows_fp64_sums": t_ap32r * 1e6,
         "apply_us_fp32_difference_form": t_ap32d * 1e6,
         "apply_us_fp32_fp32_sums": t_ap32f * 1e6,
         "apply_mdof_s_fp32_fp32_sums": 3 * nn / t_ap32f / 1e6,

@@ -18,6 +18,13 @@ namespace spg {
 __constant__ double cK0d[576];
 __constant__ float cK0f[576];
 
+// constants of the strain-form cell operator per unit modulus: 9mu, 4.5la,
+// 4.5mu, 1.5mu, 1.5la and 2mu+0.5la, all / 72 (k_apply_cell)
+struct CellConsts {
+  double d, t, s, w, q, c3;
+};
+static CellConsts g_cell;
+
 template <class T>
 struct KT;
 template <>
@@ -44,7 +51,9 @@ static int host_entry(int x, int y, int z) {
   return (sp << 8) | q;
 }
 
-void upload_k0(const double* K0, hipStream_t st) {
+void upload_k0(const double* K0, double mu, double lambda, hipStream_t st) {
+  g_cell = CellConsts{9.0 * mu / 72.0,  4.5 * lambda / 72.0, 4.5 * mu / 72.0,
+                      1.5 * mu / 72.0,  1.5 * lambda / 72.0, (2.0 * mu + 0.5 * lambda) / 72.0};
   float f[576];
   for (int i = 0; i < 576; ++i) f[i] = (float)K0[i];
   (void)st;
@@ -325,6 +334,221 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
     }
     if (MODE == 3 || MODE == 5) {
       // one partial per wavefront (= per block)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
+      if (lane == 0) dpart[blk] = d;
+    }
+  }
+}
+
+// ------------------------------------------------ cell-wise (strain-form) apply
+// K0 = mu0 K_mu + la0 K_la in the 2x2x2 Walsh basis of a cell (modes m =
+// products of +-1 per axis: the 3 gradients, 3 twists, 1 trilinear and the
+// mean, per component) is block diagonal with 45 non-zeros and no mean
+// coupling.  Written in strain form it is, per unit modulus (constants
+// below, all / 72):
+//   gradients   out(d,d) = 9 mu0 g(d,d) + 4.5 la0 tr g;
+//               out(d,v) = out(v,d) = 4.5 mu0 (g(d,v) + g(v,d))
+//   twists      the triple {(yz,x), (xz,y), (xy,z)}: 1.5 mu0 (t_i + sum t);
+//               the pairs {(yz,y),(xz,x)}, {(yz,z),(xy,x)}, {(xz,z),(xy,y)}:
+//               4.5 mu0 a + 1.5 la0 (a + b)
+//   trilinear   (2 mu0 + 0.5 la0) t3
+// and the node contributions are the transposed transform.  Every mode is
+// formed from edge differences (exact in FP32 for smooth fields), and the
+// symmetric gradient sums pair the two edge differences of each edge pair
+// before adding the pairs, so rigid translations and rotations cancel
+// exactly and FP32 sums keep the accuracy of the row sums
+// (SPGrid_Linear_Elasticity.h:22-213 computes the same product node-wise).
+// (constants of the strain-form cell operator: CellConsts, set by upload_k0)
+template <class T>
+struct CellK {
+  T d, t, s, w, q, c3;
+};
+
+// f[J][v] += (E K0 a)[J][v] for one cell (corners J = x*4+y*2+z)
+template <class T>
+__device__ __forceinline__ void cell_strain_apply(const T (&a)[8][3], const CellK<T>& k, T (&f)[8][3]) {
+  // edge differences along x (index y*2+z), y (x*2+z), z (x*2+y), per component
+  T Dx[4][3], Dy[4][3], Dz[4][3];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        Dx[p * 2 + q][v] = a[4 + p * 2 + q][v] - a[p * 2 + q][v];
+        Dy[p * 2 + q][v] = a[p * 4 + 2 + q][v] - a[p * 4 + q][v];
+        Dz[p * 2 + q][v] = a[p * 4 + q * 2 + 1][v] - a[p * 4 + q * 2][v];
+      }
+  const T gxx = (Dx[0][0] + Dx[1][0]) + (Dx[2][0] + Dx[3][0]);
+  const T gyy = (Dy[0][1] + Dy[1][1]) + (Dy[2][1] + Dy[3][1]);
+  const T gzz = (Dz[0][2] + Dz[1][2]) + (Dz[2][2] + Dz[3][2]);
+  // symmetric sums, edge pair by edge pair: y-edge (x=a,z) with x-edge (y=a,z)
+  const T Sxy = ((Dy[0][0] + Dx[0][1]) + (Dy[1][0] + Dx[1][1])) + ((Dy[2][0] + Dx[2][1]) + (Dy[3][0] + Dx[3][1]));
+  // z-edge (x=a,y) with x-edge (y,z=a)
+  const T Sxz = ((Dz[0][0] + Dx[0][2]) + (Dz[1][0] + Dx[2][2])) + ((Dz[2][0] + Dx[1][2]) + (Dz[3][0] + Dx[3][2]));
+  // z-edge (x,y=a) with y-edge (x,z=a)
+  const T Syz = ((Dz[0][1] + Dy[0][2]) + (Dz[1][1] + Dy[1][2])) + ((Dz[2][1] + Dy[2][2]) + (Dz[3][1] + Dy[3][2]));
+  T Byz[3], Bxz[3], Bxy[3], B3[3];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const T zy0 = Dz[1][v] - Dz[0][v], zy1 = Dz[3][v] - Dz[2][v];  // along y of the z-edges, x = 0 / 1
+    Byz[v] = zy0 + zy1;
+    Bxz[v] = (Dz[2][v] - Dz[0][v]) + (Dz[3][v] - Dz[1][v]);
+    Bxy[v] = (Dy[2][v] - Dy[0][v]) + (Dy[3][v] - Dy[1][v]);
+    B3[v] = zy1 - zy0;
+  }
+  // operator in the mode basis
+  const T tr = (gxx + gyy) + gzz;
+  const T ttr = k.t * tr;
+  T G[3][3];  // gradient-mode outputs G[d][v]
+  G[0][0] = fmaT(k.d, gxx, ttr);
+  G[1][1] = fmaT(k.d, gyy, ttr);
+  G[2][2] = fmaT(k.d, gzz, ttr);
+  G[0][1] = G[1][0] = k.s * Sxy;
+  G[0][2] = G[2][0] = k.s * Sxz;
+  G[1][2] = G[2][1] = k.s * Syz;
+  const T st = (Byz[0] + Bxz[1]) + Bxy[2];
+  T Oyz[3], Oxz[3], Oxy[3];
+  Oyz[0] = k.w * (Byz[0] + st);
+  Oxz[1] = k.w * (Bxz[1] + st);
+  Oxy[2] = k.w * (Bxy[2] + st);
+  {
+    const T p1 = k.q * (Byz[1] + Bxz[0]);
+    Oyz[1] = fmaT(k.s, Byz[1], p1);
+    Oxz[0] = fmaT(k.s, Bxz[0], p1);
+    const T p2 = k.q * (Byz[2] + Bxy[0]);
+    Oyz[2] = fmaT(k.s, Byz[2], p2);
+    Oxy[0] = fmaT(k.s, Bxy[0], p2);
+    const T p3 = k.q * (Bxz[2] + Bxy[1]);
+    Oxz[2] = fmaT(k.s, Bxz[2], p3);
+    Oxy[1] = fmaT(k.s, Bxy[1], p3);
+  }
+  // back to the corners: f_J = sum_m c_m(J) out_m, c_m(J) = prod_{axis in m} (J_axis ? +1 : -1)
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const T O3 = k.c3 * B3[v];
+    // along z first: per (x,y) corner pair, the modes with and without z
+    //   lo = A - Bz, hi = A + Bz with A = sx gx + sy gy + sx sy gxy, Bz = gz + sx gxz + sy gyz + sx sy O3
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const T sxv = x ? T(1) : T(-1), syv = y ? T(1) : T(-1);
+        const T A = fmaT(sxv, G[0][v], fmaT(syv, G[1][v], (x == y) ? Oxy[v] : -Oxy[v]));
+        const T Bz = fmaT(sxv, Oxz[v], fmaT(syv, Oyz[v], G[2][v] + ((x == y) ? O3 : -O3)));
+        f[x * 4 + y * 2 + 0][v] += A - Bz;
+        f[x * 4 + y * 2 + 1][v] += A + Bz;
+      }
+  }
+}
+
+// Stiffness apply cell by cell: a wavefront owns a block, stages the 6^3
+// node and 5^3 cell halos, evaluates the 125 cells around the block (two
+// passes of 64 lanes), scatters their corner forces into an LDS copy of the
+// 6^3 nodes and reads its own 64 rows.  Modes as k_apply_lds (0: K in;
+// 1: P (b - K in); 2: P K in; 3: as 2 with per-block in . out partials).
+template <class T, int MODE>
+__global__ void __launch_bounds__(64 * AP_WAVES) k_apply_cell(int nb, const int* __restrict__ nbr,
+                                                              const uint8_t* __restrict__ flags,
+                                                              const T* __restrict__ E, CellConsts kc, T dx,
+                                                              const T* __restrict__ in, const T* __restrict__ b,
+                                                              T* __restrict__ out, double* __restrict__ dpart = nullptr) {
+  __shared__ T hu[AP_WAVES][3][216];
+  __shared__ T acc[AP_WAVES][3][216];
+  __shared__ T he[AP_WAVES][128];
+  __shared__ int ht[AP_WAVES][28];
+  __shared__ int htab6[216], htab5[128];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = xcd_chunk(blockIdx.x, gridDim.x) * AP_WAVES + w;
+  const bool live = blk < nb;
+  if (lane < 27) ht[w][lane] = !live ? -1 : (lane == 13 ? blk : nbr[blk * 27 + lane]);
+  if (threadIdx.x < 216) htab6[threadIdx.x] = cHalo6[threadIdx.x];
+  if (threadIdx.x < 125) htab5[threadIdx.x] = cHalo5[threadIdx.x];
+  __syncthreads();
+  {
+    T v[4][3], e[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int h = lane + 64 * i;
+      const int t = h < 216 ? htab6[h] : 0;
+      const int bb = h < 216 ? ht[w][t >> 8] : -1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[i][c] = bb >= 0 ? in[vidx(bb, c, t & 255)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int h = lane + 64 * i;
+      const int t = h < 125 ? htab5[h] : 0;
+      const int bb = h < 125 ? ht[w][t >> 8] : -1;
+      e[i] = bb >= 0 ? E[bb * 64 + (t & 255)] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (lane + 64 * i < 216)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          hu[w][c][lane + 64 * i] = v[i][c];
+          acc[w][c][lane + 64 * i] = T(0);
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (lane + 64 * i < 125) he[w][lane + 64 * i] = e[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ci = lane + 64 * pass;
+    if (ci < 125) {
+      const int cx = ci / 25, cy = (ci / 5) % 5, cz = ci % 5;  // halo cell = halo node of its corner J = 0
+      const int n0 = cx * 36 + cy * 6 + cz;
+      const T ee = he[w][ci] * dx;
+      const CellK<T> k{ee * (T)kc.d, ee * (T)kc.t, ee * (T)kc.s, ee * (T)kc.w, ee * (T)kc.q, ee * (T)kc.c3};
+      T a[8][3], f[8][3];
+#pragma unroll
+      for (int J = 0; J < 8; ++J) {
+        const int o = ((J >> 2) & 1) * 36 + ((J >> 1) & 1) * 6 + (J & 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          a[J][c] = hu[w][c][n0 + o];
+          f[J][c] = T(0);
+        }
+      }
+      cell_strain_apply<T>(a, k, f);
+      // corner J of the 64 cells of a pass lands on 64 distinct nodes, and a
+      // wavefront's LDS instructions execute in order: a read-add-write per
+      // corner, with the compiler kept from moving one corner's accesses
+      // past another's (other lanes may touch the same node at another
+      // corner), is exact.  LDS float atomics measured 8x slower here.
+#pragma unroll
+      for (int J = 0; J < 8; ++J) {
+        const int o = ((J >> 2) & 1) * 36 + ((J >> 1) & 1) * 6 + (J & 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[w][c][n0 + o] += f[J][c];
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+      }
+    }
+  }
+  __syncthreads();
+  if (!live) return;  // wave-uniform
+  const int li = lane >> 4, lj = (lane >> 2) & 3, lk = lane & 3;
+  const int hn = (li + 1) * 36 + (lj + 1) * 6 + (lk + 1);
+  const T f[3] = {acc[w][0][hn], acc[w][1][hn], acc[w][2][hn]};
+  if (MODE == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[vidx(blk, c, lane)] = f[c];
+  } else {
+    const uint8_t fl = flags[blk * 64 + lane];
+    double d = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = MODE >= 2 ? f[c] : b[vidx(blk, c, lane)] - f[c];
+      const T o = (fl & (2 << c)) ? T(0) : v;
+      out[vidx(blk, c, lane)] = o;
+      if (MODE == 3) d += (double)hu[w][c][hn] * (double)o;
+    }
+    if (MODE == 3) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
       if (lane == 0) dpart[blk] = d;
@@ -628,10 +852,13 @@ __device__ __forceinline__ T quad_sum(T x) {
 // corner_row), the quad sums, and lane q = 0 updates the node.  A 256-thread
 // workgroup stages the 5^3 halos of its 8 blocks.  Same race argument as
 // k_gs0_colour.
+// With `dist` given, only nodes at distance >= dmin from the boundary band
+// are updated (the far part of a sweep that runs beside the band phase).
 template <class T, int QB>
 __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, const int* __restrict__ nbr,
                                                          const uint8_t* __restrict__ flags, const T* __restrict__ E,
-                                                         T dx, T* __restrict__ u, const T* __restrict__ b) {
+                                                         T dx, T* __restrict__ u, const T* __restrict__ b,
+                                                         const uint8_t* __restrict__ dist, int dmin) {
   static_assert(QB >= 4, "the 125-entry halo table is filled by one pass of the workgroup");
   __shared__ T hu[QB][3][128];
   __shared__ int ht[QB][27];
@@ -652,6 +879,7 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
   const int l = loc3(px - 1 + nx, py - 1 + ny, pz - 1 + nz);
   T ee[2], bv[3] = {T(0), T(0), T(0)};
   uint8_t fl = 0;
+  bool skip = false;
   {
     // 32 threads per block stage its 5^3 halo (4 passes, table in LDS).  The
     // halo loads, this lane's two cell moduli and (lane q = 0) the node's
@@ -675,6 +903,7 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
     }
     if (q == 0 && blk < nb) {
       fl = flags[blk * 64 + l];
+      if (dist) skip = dist[blk * 64 + l] < dmin;
 #pragma unroll
       for (int c = 0; c < 3; ++c) bv[c] = b[vidx(blk, c, l)];
     }
@@ -711,7 +940,7 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
   f2 = dx * quad_sum(f2);
   const T S = dx * quad_sum(es), Sxy = dx * quad_sum(sy * ed), Sxz = dx * quad_sum(sz * ed),
           Syz = dx * quad_sum(sy * sz * es);
-  if (q != 0 || blk >= nb) return;
+  if (q != 0 || blk >= nb || skip) return;
   T d[6] = {KT<T>::k(0) * S, KT<T>::k(4) * S, KT<T>::k(8) * S, KT<T>::k(1) * Sxy, KT<T>::k(2) * Sxz, KT<T>::k(5) * Syz};
   const int hn = (nx * 5 + ny) * 5 + nz;
   gs_node_update_b<T>(fl, d, f0, f1, f2, blk, l, H[hn], H[128 + hn], H[256 + hn], bv[0], bv[1], bv[2], u);
@@ -2427,9 +2656,36 @@ static bool diff_sums() {
   const char* e = std::getenv("SPG_DIFF_SUMS");
   return e && std::atoi(e) == 1;
 }
+// Which applies run cell by cell in strain form (k_apply_cell): by default
+// the FP32 Krylov operator P K s (modes 2 and 3), whose FP32 row sums would
+// otherwise need FP64 accumulation; SPG_APPLY_CELL=0 none, 1 every apply.
+// 256^3 shell, per apply: FP32 operator 121 us (cell) vs 162 us (rows with
+// FP64 sums); FP32 residual 116 vs 107 us and FP64 228 vs 179 us favour the
+// rows there.
+static bool apply_cell(bool fp32, int mode) {
+  const char* e = std::getenv("SPG_APPLY_CELL");
+  if (e) return std::atoi(e) == 1;
+  return fp32 && mode >= 2;
+}
+template <class T>
+static void launch_apply_cell(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out,
+                              int mode, double* dpart = nullptr) {
+  const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
+  const CellConsts k = g_cell;
+  if (mode == 0)
+    k_apply_cell<T, 0><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, k, dx, in, b, out);
+  else if (mode == 1)
+    k_apply_cell<T, 1><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, k, dx, in, b, out);
+  else if (mode == 2)
+    k_apply_cell<T, 2><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, k, dx, in, b, out);
+  else
+    k_apply_cell<T, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, k, dx, in, b, out, dpart);
+}
 template <class T>
 void launch_apply(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out, int mode) {
-  if (sizeof(T) == 4 && mode == 2 && diff_sums()) {
+  if (apply_cell(sizeof(T) == 4, mode)) {
+    launch_apply_cell<T>(st, L, E, dx, in, b, out, mode);
+  } else if (sizeof(T) == 4 && mode == 2 && diff_sums()) {
     const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
     k_apply_lds<T, T, 4><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, b, out);
   } else if (sizeof(T) == 4 && ((acc64_modes() >> mode) & 1)) {
@@ -2445,7 +2701,9 @@ void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const
   // 256^3 shell, 18 iterations: mixed 42.8 vs 43.1 ms, FP64 60.0 vs 60.5 ms
   // against the apply followed by a separate dot
   const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
-  if (sizeof(T) == 4 && diff_sums())
+  if (apply_cell(sizeof(T) == 4, 3))
+    launch_apply_cell<T>(st, L, E, dx, in, nullptr, out, 3, bpart);
+  else if (sizeof(T) == 4 && diff_sums())
     k_apply_lds<T, T, 5><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
   else if (sizeof(T) == 4 && ((acc64_modes() >> 2) & 1))
     k_apply_lds<T, double, 3><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, dx, in, nullptr, out, bpart);
@@ -2645,7 +2903,13 @@ void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, dou
 // k_gs0_colour, 2 the corner-per-wave k_gs0_colour_uc, 3 the quad-lane
 // k_gs0_colour4; 0 or unset picks per precision.  All are parity-tested.
 template <class T>
-void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour) {
+void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour,
+                const uint8_t* dist, int dmin) {
+  if (dist) {
+    // the far part of a sweep beside the band phase: the quad kernel
+    k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, dist, dmin);
+    return;
+  }
   const char* e = std::getenv("SPG_GS_KERNEL");
   // measured on gfx950 (256^3 shell, per 8-colour sweep): quad 374 / 535 us,
   // octet 455 / 580 us, corner-per-wave 533 / 898 us (FP32 / FP64); two
@@ -2662,11 +2926,14 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
     const char* q = std::getenv("SPG_GS_QB");
     const int qb = q ? std::atoi(q) : 8;
     if (qb == 4)
-      k_gs0_colour4<T, 4><<<(L.nb + 3) / 4, 128, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+      k_gs0_colour4<T, 4><<<(L.nb + 3) / 4, 128, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
+                                                                  0);
     else if (qb == 16)
-      k_gs0_colour4<T, 16><<<(L.nb + 15) / 16, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+      k_gs0_colour4<T, 16><<<(L.nb + 15) / 16, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
+                                                                  0);
     else
-      k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
+      k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
+                                                                  0);
   } else {
     const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
@@ -2836,6 +3103,14 @@ void launch_dense_assemble(hipStream_t st, const LevelDev& L, const double* ST, 
                            double* A) {
   k_dense_assemble<<<nblk((int64_t)L.nb * 64), TPB, 0, st>>>(L.nb, L.nbr, L.flags, ST, var_of, n, A);
 }
+void launch_list_nslot(hipStream_t st, const LevelDev& L, int n, const int* list, int* out) {
+  if (n > 0) k_bnd_nslot<<<nblk((int64_t)n * 27), TPB, 0, st>>>(n, list, L.nbr, out);
+}
+template <class T>
+void launch_gs0_list(hipStream_t st, const LevelDev& L, const int* list, const int* nslot, int n, const T* E, T dx,
+                     T* u, const T* b) {
+  if (n > 0) k_gs0_list_tab<T><<<(n + 31) / 32, 256, 0, st>>>(list, nslot, n, L.flags, E, dx, u, b);
+}
 void launch_bnd_nslot(hipStream_t st, const LevelDev& L, int* out) {
   const int n = L.bnd_off[8];
   if (n > 0) k_bnd_nslot<<<nblk((int64_t)n * 27), TPB, 0, st>>>(n, L.bnd, L.nbr, out);
@@ -2895,7 +3170,9 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_apply<T>(hipStream_t, const LevelDev&, const T*, T, const T*, const T*, T*, int);         \
   template void launch_apply_dot<T>(hipStream_t, const LevelDev&, const T*, T, const T*, T*, double*, double*,   \
                                     double*, CgScalars*, int);                                                  \
-  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int);                     \
+  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, const uint8_t*, int); \
+  template void launch_gs0_list<T>(hipStream_t, const LevelDev&, const int*, const int*, int, const T*, T, T*,      \
+                                   const T*);                                                                       \
   template void launch_gs0_boundary<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);      \
   template void launch_st_residual<T>(hipStream_t, const LevelDev&, const T*, const T*, const T*, T*);           \
   template void launch_st_gs<T>(hipStream_t, const LevelDev&, const T*, T*, const T*, int);                      \

@@ -83,7 +83,7 @@ struct CgScalars {
   int pad;
 };
 
-void upload_k0(const double* K0, hipStream_t st);
+void upload_k0(const double* K0, double mu, double lambda, hipStream_t st);
 
 // level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
 template <class T>
@@ -96,7 +96,14 @@ void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const
                       double* part, double* dot, CgScalars* sc = nullptr, int stage = -1);
 // level-0 GS: one colour over all blocks; all boundary sweeps of one phase
 template <class T>
-void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour);
+void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour,
+                const uint8_t* dist = nullptr, int dmin = 0);
+// one level-0 colour step over a list of slots (27 neighbour slots each in nslot)
+template <class T>
+void launch_gs0_list(hipStream_t st, const LevelDev& L, const int* list, const int* nslot, int n, const T* E, T dx,
+                     T* u, const T* b);
+// neighbour slots (27 per node) of a list of level slots
+void launch_list_nslot(hipStream_t st, const LevelDev& L, int n, const int* list, int* out);
 template <class T>
 void launch_gs0_boundary(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int sweeps,
                          bool forward);

@@ -223,6 +223,63 @@ int64_t vector_index_limit() {
   return INT32_MAX;
 }
 
+// colour of step i of a level-0 sweep: 0..7 forward, 7..0 backward (the
+// order of SPGrid_Gauss_Seidel.h:512-530; sweep_colour in kernels.hpp)
+static int sweep_colour0(bool forward, int i) { return forward ? i : 7 - i; }
+
+// Level-0 slots' distance from the boundary band and the near-band lists of
+// every colour step.  A sweep's step-s update at a slot farther than s + 1
+// from the band reads only values that the boundary phase before the sweep
+// does not write, directly or through earlier steps of the same sweep
+// (each earlier step brings the dependence at most one node closer), so the
+// solver may run those updates while the boundary phase is still going.
+static void band_distances(Layout& out) {
+  if (out.lv.empty()) return;
+  LevelDesc& D = out.lv[0];
+  const int64_t ns = D.nb * 64;
+  D.gs_dist.assign(ns, (uint8_t)kGsDistCap);
+  std::vector<int32_t> front;
+  for (int c = 0; c < 8; ++c)
+    for (int32_t s : D.bnd_list[c]) {
+      D.gs_dist[s] = 0;
+      front.push_back(s);
+    }
+  for (int d = 1; d < kGsDistCap && !front.empty(); ++d) {
+    std::vector<int32_t> next;
+    for (int32_t s : front) {
+      const int64_t g = s >> 6;
+      const int q = s & 63, li = q >> 4, lj = (q >> 2) & 3, lk = q & 3;
+      for (int sp = 0; sp < 27; ++sp) {
+        if (sp == 13) continue;
+        const int a = li + sp / 9 - 1, b = lj + (sp / 3) % 3 - 1, c = lk + sp % 3 - 1;
+        const int rs = (a < 0 ? 0 : (a > 3 ? 2 : 1)) * 9 + (b < 0 ? 0 : (b > 3 ? 2 : 1)) * 3 + (c < 0 ? 0 : (c > 3 ? 2 : 1));
+        const int32_t nb = rs == 13 ? (int32_t)g : D.nbr[(size_t)g * 27 + rs];
+        if (nb < 0) continue;
+        const int32_t t = nb * 64 + (((a & 3) << 4) | ((b & 3) << 2) | (c & 3));
+        if (D.gs_dist[t] > d) {
+          D.gs_dist[t] = (uint8_t)d;
+          next.push_back(t);
+        }
+      }
+    }
+    front.swap(next);
+  }
+  for (int dir = 0; dir < 2; ++dir)
+    for (int st = 0; st < 8; ++st) {
+      const int colour = sweep_colour0(dir == 0, st);
+      std::vector<int32_t>& L = D.gs_near[dir][st];
+      L.clear();
+      if (D.gs_dist.empty()) continue;
+      for (int64_t g = 0; g < D.nb; ++g)
+        for (int k = 0; k < 8; ++k) {
+          const int q = ((((colour >> 2) & 1) + 2 * ((k >> 2) & 1)) << 4) |
+                        ((((colour >> 1) & 1) + 2 * ((k >> 1) & 1)) << 2) | ((colour & 1) + 2 * (k & 1));
+          const int32_t s = (int32_t)(g * 64 + q);
+          if (D.gs_dist[s] <= st + 1) L.push_back(s);
+        }
+    }
+}
+
 Layout build_layout(const ProblemDesc& p) {
   Layout out;
   if (p.nblocks <= 0) throw LayoutError("no density blocks", -1);
@@ -597,6 +654,7 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
   }
+  band_distances(out);
   return out;
 }
 

@@ -63,6 +63,9 @@ struct ProblemDesc {
 // largest level vector (entries) the device kernels can index with int
 int64_t vector_index_limit();
 
+// distances from the level-0 boundary band are kept up to this value
+constexpr int kGsDistCap = 9;
+
 // SIMP map (topo_opt/fem_interface.h:343-353)
 double stiffness_mapping(double penalty, double min_stiffness, double min_density, double d);
 
@@ -147,6 +150,13 @@ struct LevelDesc {
   // (-1 outside the boundary set)
   std::vector<int32_t> bnd_cslot;
   std::vector<int16_t> bnd_cnode;
+  // level 0: distance of every slot from the boundary band (26-neighbour
+  // steps, capped at kGsDistCap) and, per sweep direction (0 forward, 1
+  // backward) and colour step s, the slots of that step's colour within
+  // distance s + 1 of the band: the only ones whose update can depend on the
+  // boundary phase that precedes the sweep
+  std::vector<uint8_t> gs_dist;
+  std::vector<int32_t> gs_near[2][8];
   int64_t n_active = 0;
   bool is_explicit = false;
   // explicit levels: Galerkin data

@@ -91,12 +91,16 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   size_t free0 = 0, total0 = 0;
   HIPCHK(hipMemGetInfo(&free0, &total0));
   HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ev_a_, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_b_, hipEventDisableTiming));
+  if (const char* e = std::getenv("SPG_OVERLAP")) overlap_ = std::atoi(e) != 0;
   RBCHK(rocblas_create_handle(&rb_));
   RBCHK(rocblas_set_stream(rb_, st_));
   double kmu[576], kla[576];
   element_tables(kmu, kla);
   for (int e = 0; e < 576; ++e) K0_[e] = cfg.mu * kmu[e] + cfg.lambda * kla[e];
-  upload_k0(K0_, st_);
+  upload_k0(K0_, cfg.mu, cfg.lambda, st_);
   upload_layout();
   build_operators();
   sync();
@@ -109,6 +113,9 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
 Solver::~Solver() {
   if (rb_) rocblas_destroy_handle(rb_);
   if (hscal_) (void)hipHostFree(hscal_);
+  if (ev_a_) (void)hipEventDestroy(ev_a_);
+  if (ev_b_) (void)hipEventDestroy(ev_b_);
+  if (st2_) (void)hipStreamDestroy(st2_);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -190,6 +197,27 @@ void Solver::upload_layout() {
       R.bnslot.alloc(bnd.size() * 27 * 4);
       launch_bnd_nslot(st_, R.d, R.bnslot.as<int>());
       R.d.bnslot = R.bnslot.as<int>();
+    }
+    if (l == 0 && !bnd.empty() && !D.gs_dist.empty()) {
+      // near-band step lists of both sweep directions, and their neighbour slots
+      std::vector<int32_t> all;
+      for (int i = 0; i < 16; ++i) {
+        const std::vector<int32_t>& L = D.gs_near[i / 8][i % 8];
+        R.gsnear_off[i] = (int64_t)all.size();
+        if (!L.empty()) {
+          // the layout's step colours must be the sweep's
+          const int32_t s0 = L[0] & 63;
+          const int colour = (((s0 >> 4) & 1) << 2) | (((s0 >> 2) & 1) << 1) | (s0 & 1);
+          if (colour != sweep_colour(true, i < 8, i % 8))
+            throw SpgError(SPG_E_INVALID, "near-band step list in the wrong colour order");
+        }
+        all.insert(all.end(), L.begin(), L.end());
+      }
+      R.gsnear_off[16] = (int64_t)all.size();
+      upload(R.gsdist, D.gs_dist);
+      upload(R.gsnear, all);
+      R.gsnslot.alloc(std::max<size_t>(all.size(), 1) * 27 * 4);
+      launch_list_nslot(st_, R.d, (int)all.size(), R.gsnear.as<int>(), R.gsnslot.as<int>());
     }
     R.d.flags = R.flags.as<uint8_t>();
     R.d.base = R.base.as<int>();
@@ -496,8 +524,34 @@ void Solver::smooth(int l, T* u, const T* b, bool forward) {
     if (l == 0) launch_gs0_boundary<T>(st_, R.d, E<T>(), dxT<T>(), u, b, cfg_.boundary_iters, forward);
     else launch_st_boundary<T>(st_, R.d, R.st<T>(), u, b, cfg_.boundary_iters, forward);
   };
-  boundary();
-  for (int i = 0; i < cfg_.smooth_iters; ++i)
+  int first = 0;
+  if (l == 0 && overlap_ && R.gsdist.p && cfg_.smooth_iters > 0 && cfg_.boundary_iters > 0) {
+    // The boundary phase writes only band nodes.  The first sweep's step-c
+    // update of a node farther than c + 1 from the band reads nothing that
+    // phase writes, directly or through earlier steps (Layout::gs_near), so
+    // those updates run on st2_ beside the phase; the near ones follow on
+    // st_ once both are done, step by step as the sequential sweep orders
+    // them.  The result is the sequential sweep's.  256^3 shell, FP32
+    // V-cycle: 1.71 ms against 1.62 ms sequential (the near-band list
+    // launches cost what the overlap saves), hence off by default.
+    HIPCHK(hipEventRecord(ev_a_, st_));
+    boundary();
+    HIPCHK(hipStreamWaitEvent(st2_, ev_a_, 0));
+    for (int c = 0; c < 8; ++c)
+      launch_gs0<T>(st2_, R.d, E<T>(), dxT<T>(), u, b, sweep_colour(true, forward, c), R.gsdist.as<uint8_t>(), c + 2);
+    HIPCHK(hipEventRecord(ev_b_, st2_));
+    HIPCHK(hipStreamWaitEvent(st_, ev_b_, 0));
+    const int dir = forward ? 0 : 1;
+    for (int c = 0; c < 8; ++c) {
+      const int64_t o = R.gsnear_off[dir * 8 + c], n = R.gsnear_off[dir * 8 + c + 1] - o;
+      launch_gs0_list<T>(st_, R.d, R.gsnear.as<int>() + o, R.gsnslot.as<int>() + o * 27, (int)n, E<T>(), dxT<T>(), u,
+                         b);
+    }
+    first = 1;
+  } else {
+    boundary();
+  }
+  for (int i = first; i < cfg_.smooth_iters; ++i)
     for (int c = 0; c < 8; ++c) {
       if (l == 0) launch_gs0<T>(st_, R.d, E<T>(), dxT<T>(), u, b, sweep_colour(true, forward, c));
       else launch_st_gs<T>(st_, R.d, R.st<T>(), u, b, sweep_colour(false, forward, c));

@@ -54,6 +54,8 @@ struct LevelRT {
   DBuf b0ub, coop;  // cooperative level-0 phase: iterate and {barrier counter, error word}
   DBuf bmap64[2], bmap32[2], bmx64, bmx32;           // dense boundary maps (explicit levels)
   DBuf bnslot;                                       // level 0: neighbour slots of boundary nodes
+  DBuf gsdist, gsnear, gsnslot;                      // level 0: band distances, near-band step lists, their slots
+  int64_t gsnear_off[17] = {};                       // list (dir * 8 + step) = gsnear[off[i] .. off[i+1])
   DBuf u32, b32, r32, u64, b64, r64;
   int64_t n3 = 0;  // vector length nb*192
   template <class T> T* u() const;
@@ -126,6 +128,11 @@ class Solver {
   std::vector<LevelRT> lv_;
   double K0_[576];
   hipStream_t st_ = nullptr;
+  // SPG_OVERLAP=1: the far part of a level-0 sweep runs on st2_ beside the
+  // boundary phase on st_ (off by default: measured slower)
+  hipStream_t st2_ = nullptr;
+  hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+  bool overlap_ = false;
   rocblas_handle rb_ = nullptr;
   // level-0 data
   DBuf E64_, E32_, rho_, in_slot_, ud_idx_, ud_val_, f_idx_, f_val_;

@@ -95,11 +95,29 @@ def test_apply_and_residual(blocky, prec, tol):
     assert rel(got, P.from_o(0, want_full)) < tol * 10
 
 
+def _fp32_operator_forms(P, u, monkeypatch):
+    """relative error of every FP32 form of the Krylov operator P K u against
+    the FP64 operator on the same (FP32-rounded) input"""
+    want = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=1)
+    errs = {"cell (default)": rel(P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0), want)}
+    monkeypatch.setenv("SPG_APPLY_CELL", "0")
+    errs["rows, fp64 sums"] = rel(P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0), want)
+    monkeypatch.setenv("SPG_DIFF_SUMS", "1")
+    errs["rows, difference form"] = rel(P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0), want)
+    monkeypatch.setenv("SPG_DIFF_SUMS", "0")
+    monkeypatch.setenv("SPG_ACC64_MODES", "0")
+    errs["rows, fp32 sums"] = rel(P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0), want)
+    for k in ("SPG_APPLY_CELL", "SPG_DIFF_SUMS", "SPG_ACC64_MODES"):
+        monkeypatch.delenv(k)
+    return errs
+
+
 def test_fp32_operator_difference_form_on_smooth_fields(blocky, monkeypatch):
     """the FP32 Krylov operator P K u on a smooth field, where direct FP32 row
-    sums cancel large terms: both cures (FP64 row sums, the default; the
-    difference form K (u_m - u_n) with FP32 sums, SPG_DIFF_SUMS=1) stay far
-    below the direct FP32 form's error against the FP64 apply"""
+    sums cancel large terms: the cures (the strain-form cell apply, the
+    default; node rows with FP64 sums; the difference form K (u_m - u_n)
+    with FP32 sums) stay far below the direct FP32 form's error against the
+    FP64 apply"""
     P = blocky
     gx, gf = P.g.level_nodes(0)
     x = gx.astype(np.float64)
@@ -108,17 +126,58 @@ def test_fp32_operator_difference_form_on_smooth_fields(blocky, monkeypatch):
     # direct sums cancel terms of the translation's size
     u = 1.0 + 1e-3 * np.sin(x / 7.0 + np.array([0.0, 1.0, 2.0]))
     u = u.astype(np.float32).astype(np.float64)  # the FP64 reference sees the same FP32 input
-    want = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=1)
-    got_fp64sums = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)  # the default
-    monkeypatch.setenv("SPG_DIFF_SUMS", "1")
-    got_diff = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)
-    monkeypatch.setenv("SPG_DIFF_SUMS", "0")
-    monkeypatch.setenv("SPG_ACC64_MODES", "0")
-    got_direct = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=0)
-    e64, e_diff, e_direct = rel(got_fp64sums, want), rel(got_diff, want), rel(got_direct, want)
-    print(f"smooth field: FP64 sums {e64:.2e}, difference form {e_diff:.2e}, direct FP32 sums {e_direct:.2e}")
-    assert e64 < 1e-5 and e_diff < 1e-4
-    assert e64 < 0.05 * e_direct and e_diff < 0.05 * e_direct
+    errs = _fp32_operator_forms(P, u, monkeypatch)
+    print("smooth field:", {k: f"{v:.2e}" for k, v in errs.items()})
+    direct = errs["rows, fp32 sums"]
+    for k in ("cell (default)", "rows, fp64 sums", "rows, difference form"):
+        assert errs[k] < 1e-4 and errs[k] < 0.05 * direct, k
+    assert errs["rows, fp64 sums"] < 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 2e-6)])
+def test_cell_apply_matches_oracle(blocky, prec, tol, monkeypatch):
+    """the strain-form cell apply (SPG_APPLY_CELL=1): K u, P (b - K u) and the
+    Krylov operator P K u against the oracle's apply"""
+    P = blocky
+    monkeypatch.setenv("SPG_APPLY_CELL", "1")
+    rng = np.random.default_rng(1)
+    n = len(P.map[0][0])
+    u = rng.standard_normal((n, 3))
+    b = rng.standard_normal((n, 3))
+    Ku = P.o.apply(P.to_o(0, u))
+    dm = ((P.o.flags(0)[:, None] >> (5 + np.arange(3))) & 1).astype(bool)
+    got = P.g.kernel(_lib.OP_APPLY, 0, u, precision=prec)
+    assert rel(got, P.from_o(0, Ku)) < tol
+    want = P.to_o(0, b) - Ku
+    want[dm] = 0
+    got = P.g.kernel(_lib.OP_RESIDUAL, 0, u, b, precision=prec)
+    assert rel(got, P.from_o(0, want)) < tol * 10
+    want = Ku.copy()
+    want[dm] = 0
+    got = P.g.kernel(_lib.OP_OPERATOR, 0, u, precision=prec)
+    assert rel(got, P.from_o(0, want)) < tol
+
+
+def test_fp32_cell_apply_on_rotating_fields(blocky, monkeypatch):
+    """a rigid rotation plus a translation and a small long-wave part, all
+    rounded to FP32: the strain-form cell apply with FP32 sums (rotations and
+    translations cancel within each cell) against the FP64 apply of the same
+    input, beside the node-wise forms (FP64 row sums; difference form;
+    direct FP32 sums).  The cell form's remaining error is the FP32 rounding
+    of each cell's stresses, which cancel between neighbouring cells (about
+    2e-5 here, measured also in a NumPy FP32 emulation of the kernel)."""
+    P = blocky
+    gx, gf = P.g.level_nodes(0)
+    x = gx.astype(np.float64)
+    x -= x.mean(0)
+    om = np.array([0.3, -0.7, 0.5])
+    u = np.cross(om[None, :], x) + np.array([5.0, -3.0, 2.0]) + 1e-3 * np.sin(x / 7.0 + np.array([0.0, 1.0, 2.0]))
+    u = u.astype(np.float32).astype(np.float64)
+    errs = _fp32_operator_forms(P, u, monkeypatch)
+    print("rotating field:", {k: f"{v:.2e}" for k, v in errs.items()})
+    cell = errs["cell (default)"]
+    assert cell < 1e-4 and errs["rows, fp64 sums"] < 1e-6
+    assert cell < 0.5 * errs["rows, difference form"] and cell < 0.05 * errs["rows, fp32 sums"]
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 5e-5)])
@@ -269,6 +328,41 @@ def test_bottom_solve_is_exact(blocky):
     x = P.g.kernel(_lib.OP_BOTTOM, B, None, b, precision=1)
     res = P.o.stencil_residual(B, P.to_o(B, x), P.to_o(B, b))
     assert np.abs(res).max() < 1e-9 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 2e-5)])
+@pytest.mark.parametrize("geometry", ["blocky", "clamped_face"])
+def test_overlapped_level0_smoothing_matches_sequential(geometry, prec, tol, monkeypatch):
+    """the far part of the first level-0 sweep beside the boundary phase, the
+    near-band part after it (SPG_OVERLAP=1) against the plain sequence
+    (SPG_OVERLAP=0, the default): the same V-cycle up to the summation order of
+    the near-band rows, and the same solve (measured: FP64 V-cycles agree to
+    3e-17, FP32 to 6e-8)"""
+    if geometry == "blocky":
+        prob = W.random_blocky(seed=8, n=(24, 16, 16), mg_levels=3)
+    else:
+        prob = W.box(48, 24, 24, seed=2, mg_levels=3, load="edge")
+    rng = np.random.default_rng(17)
+    out = {}
+    for ov in ("0", "1"):
+        monkeypatch.setenv("SPG_OVERLAP", ov)
+        g = _lib.Solver(prob)
+        gx, gf = g.level_nodes(0)
+        b = rng.standard_normal((len(gx), 3)) if ov == "0" else out["b"]
+        b = b.copy()
+        b[(gf & 1) == 0] = 0
+        b[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+        z = g.kernel(_lib.OP_VCYCLE, 0, None, b, precision=prec)
+        _, info = g.solve(tolerance=1e-8, max_iterations=500, precision=prec, max_passes=30)
+        out["b"] = b
+        out[ov] = (z, info["iterations"])
+        g.close()
+    print(f"[overlap] {geometry}: V-cycle difference {rel(out['1'][0], out['0'][0]):.2e}, "
+          f"iterations {out['0'][1]} / {out['1'][1]}", flush=True)
+    assert rel(out["1"][0], out["0"][0]) < tol
+    # FP64: the same iterations; FP32: the near-band rows' other summation
+    # order may move the rounding by one iteration
+    assert abs(out["1"][1] - out["0"][1]) <= (0 if prec == 1 else 2)
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-10), (0, 1e-4)])
