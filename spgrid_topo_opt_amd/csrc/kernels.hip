@@ -2916,7 +2916,17 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
   // more layouts were measured against the quad's later 302 / 445 us and
   // dropped: a lane per node over 8 blocks per wavefront (456 / 737 us,
   // 3 waves per SIMD leave the halo loads exposed) and two lanes per node
-  // with the x-mirror split (315-352 / 541-653 us)
+  // with the x-mirror split (315-352 / 541-653 us).  Against the quad's
+  // 297-311 / 451-482 us, also measured and dropped: the quad with 16% fewer
+  // VALU instructions (precomputed LDS offsets, one address per halo entry)
+  // 311-318 / 466-477 us; the same with b and flags read from a colour-major
+  // copy (29% fewer fetched bytes) 310 / 469 us; stored inverse self blocks
+  // 359 / 571 us; a persistent, software-pipelined quad kernel with buffer
+  // loads 389 / 600 us (118 VGPRs: 4 waves per SIMD); a cell corner per
+  // wavefront with table-driven staging 436 / 726 us.  The quad kernel keeps
+  // the VALUs about 94% busy, and with fewer VALU instructions the latency of
+  // its two dependent load phases (neighbour rows, then the halo) sets the
+  // same time.
   const int kind = e && std::atoi(e) > 0 ? std::atoi(e) : 3;
   if (kind == 2) {
     k_gs0_colour_uc<T><<<(L.nb + UC_BLOCKS - 1) / UC_BLOCKS, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);

@@ -218,6 +218,41 @@ def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
         assert rel(got, want) < tol
 
 
+@pytest.mark.parametrize("host", [True, False])
+def test_gs_sweep_follows_a_density_update(host):
+    """density updates (host and device) reach the level-0 colour sweeps and
+    their boundary phase, whose inverse self blocks and boundary moduli are
+    derived from the moduli: sweeps after the update match the oracle built
+    on the new densities"""
+    import dataclasses
+
+    import torch
+
+    p = W.random_blocky(seed=11, n=(16, 12, 12), mg_levels=4)
+    P = Pair(p)
+    rng = np.random.default_rng(8)
+    d2 = np.where(p.density > 0, rng.uniform(0.05, 1.0, p.density.shape), 0.0)
+    if host:
+        P.g.update_density(d2)
+    else:
+        dd = torch.tensor(np.asarray(d2, np.float64).reshape(-1), device="cuda")
+        P.g.update_density_device(dd.data_ptr())
+        torch.cuda.synchronize()
+    o2 = O.Oracle(dataclasses.replace(p, density=d2))
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    u = rng.standard_normal((n, 3)) * 1e-6
+    u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    b = rng.standard_normal((n, 3))
+    for prec, tol in ((1, 1e-11), (0, 5e-5)):
+        for fwd in (True, False):
+            for op, bnd in ((_lib.OP_GS_SWEEP, False), (_lib.OP_GS_BOUNDARY, True)):
+                got = P.g.kernel(op, 0, u, b, precision=prec, forward=fwd)
+                want = P.from_o(0, o2.gs_sweep(0, P.to_o(0, u), P.to_o(0, b), fwd, bnd))
+                assert rel(got, want) < tol, (prec, fwd, bnd)
+    P.g.close()
+
+
 @pytest.mark.parametrize("path", ["1", "2", "3", "4", "5"])
 def test_boundary_fallback_paths(blocky, path, monkeypatch):
     """every boundary smoother (1: one workgroup from global memory; 2:
