@@ -443,7 +443,8 @@ def opt_loop(args):
             r["step_s"] = time.perf_counter() - ti
             recs.append(r)
             log(f"[opt] {name} it {it}: c={r['compliance']:.6e} step {r['step_s']:.2f}s solve {r['solve_s']:.3f}s "
-                f"pcg {r['iterations']} fem {r['fem_s']:.2f}s band {r['band_s']:.2f}s filter+oc "
+                f"pcg {r['iterations']} fem {r['fem_s']:.2f}s band {r['band_update_s']:.2f}s rebuild "
+                f"{r['rebuild_s']:.2f}s filter+oc "
                 f"{r['filter_oc_s']:.2f}s blocks {r['fem_blocks']} fat {r['fat_blocks']} rebuilt {r['rebuilt']}")
         total = time.perf_counter() - t0
         runs[name] = {
@@ -453,6 +454,9 @@ def opt_loop(args):
             "solve_s_mean": sum(r["solve_s"] for r in recs) / len(recs),
             "pcg_iterations": [r["iterations"] for r in recs],
             "rebuilds": sum(bool(r["rebuilt"]) for r in recs),
+            "band_update_s_mean": sum(r["band_update_s"] for r in recs) / len(recs),
+            "rebuild_s_mean": sum(r["rebuild_s"] for r in recs if r["rebuilt"]) / max(1, sum(bool(r["rebuilt"]) for r in recs)),
+            "fem_s_mean": sum(r["fem_s"] for r in recs) / len(recs),
             "final_fem_blocks": recs[-1]["fem_blocks"],
             "compliance_history": [r["compliance"] for r in recs],
         }

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <exception>
 #include <thread>
+#include <cstdio>
+#include <chrono>
 #include <unordered_set>
 
 namespace spg {
@@ -119,9 +121,9 @@ static inline int loc(int x, int y, int z) { return ((x & 3) << 4) | ((y & 3) <<
 // f(begin, end) over [0, n) on up to 16 threads (blocks are independent in
 // every loop that uses it); the first exception is rethrown after the join
 template <class F>
-static void parallel_ranges(int64_t n, F&& f) {
+static void parallel_ranges(int64_t n, F&& f, int64_t min_chunk = 4096) {
   const unsigned hw = std::thread::hardware_concurrency();
-  const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::min(hw, 16u)), (n + 4095) / 4096);
+  const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::min(hw, 16u)), (n + min_chunk - 1) / min_chunk);
   if (nt <= 1) {
     f((int64_t)0, n);
     return;
@@ -144,26 +146,51 @@ static void parallel_ranges(int64_t n, F&& f) {
     if (e) std::rethrow_exception(e);
 }
 
-// Flag_Nodes (Flaging.h:127-142): cells of allocated blocks push their
-// ACTIVE / DIRICHLET bits (>>4) to their 8 nodes.
+// Flag_Nodes (Flaging.h:127-142): every node takes the ACTIVE / DIRICHLET
+// bits (>>4) of the cells of allocated blocks that have it as a corner.  The
+// blocks that receive bits are created first (serially: a cell reaches the +
+// neighbours on the axes where it sits on the block's upper face); the bits
+// are then gathered node by node, so blocks are independent.
 static void flag_nodes(HostLevel& H) {
-  size_t nb0 = H.alloc.size();
+  const size_t nb0 = H.alloc.size();
   for (size_t b = 0; b < nb0; ++b) {
     if (!H.alloc[b]) continue;
-    int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
-    int32_t nb8[8] = {-2, -2, -2, -2, -2, -2, -2, -2};  // this block and its + neighbours, looked up once
+    unsigned upper = 0;  // axes (x 4, y 2, z 1) on whose upper face a flagged cell sits
+    bool any = false;
     for (int l = 0; l < 64; ++l) {
-      uint16_t nf = (uint16_t)((H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL)) >> 4);
-      if (!nf) continue;
-      int x = bx * 4 + (l >> 4), y = by * 4 + ((l >> 2) & 3), z = bz * 4 + (l & 3);
-      for (int c = 0; c < 8; ++c) {
-        int X = x + ((c >> 2) & 1), Y = y + ((c >> 1) & 1), Z = z + (c & 1);
-        const int o = (((X >> 2) - bx) << 2) | (((Y >> 2) - by) << 1) | ((Z >> 2) - bz);
-        if (nb8[o] == -2) nb8[o] = H.get(X >> 2, Y >> 2, Z >> 2);
-        H.nflag[(size_t)nb8[o] * 64 + loc(X, Y, Z)] |= nf;
+      if (!(H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+      any = true;
+      upper |= (((l >> 4) == 3) << 2) | ((((l >> 2) & 3) == 3) << 1) | ((l & 3) == 3);
+    }
+    if (!any) continue;
+    const int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
+    for (unsigned o = 1; o < 8; ++o)
+      if ((o & upper) == o) H.get(bx + ((o >> 2) & 1), by + ((o >> 1) & 1), bz + (o & 1));
+  }
+  const int64_t nb1 = (int64_t)H.alloc.size();
+  parallel_ranges(nb1, [&](int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; ++b) {
+      const int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
+      int32_t src[8];  // this block and its - neighbours (bit set: shifted by -1 on that axis), if allocated
+      bool any = false;
+      for (int o = 0; o < 8; ++o) {
+        const int32_t t = o == 0 ? (int32_t)b : H.find(bx - ((o >> 2) & 1), by - ((o >> 1) & 1), bz - (o & 1));
+        src[o] = (t >= 0 && H.alloc[t]) ? t : -1;
+        any |= src[o] >= 0;
+      }
+      if (!any) continue;
+      for (int l = 0; l < 64; ++l) {
+        const int i = l >> 4, j = (l >> 2) & 3, k = l & 3;
+        uint16_t acc = 0;
+        for (int c = 0; c < 8; ++c) {  // the cell whose corner c is this node
+          const int ci = i - ((c >> 2) & 1), cj = j - ((c >> 1) & 1), ck = k - (c & 1);
+          const int32_t t = src[((ci < 0) << 2) | ((cj < 0) << 1) | (ck < 0)];
+          if (t >= 0) acc |= H.cflag[(size_t)t * 64 + loc(ci, cj, ck)];
+        }
+        H.nflag[(size_t)b * 64 + l] |= (uint16_t)((acc & (F_ACTIVE_CELL | F_DIR_CELL)) >> 4);
       }
     }
-  }
+  });
 }
 
 // Flag_Boundary_Nodes with band 3 (Flaging.h:155-184), evaluated as a
@@ -280,7 +307,26 @@ static void band_distances(Layout& out) {
     }
 }
 
+// SPG_SETUP_TRACE=2: seconds per phase of build_layout on stderr
+namespace {
+struct PhaseTrace {
+  bool on = false;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  PhaseTrace() {
+    const char* e = std::getenv("SPG_SETUP_TRACE");
+    on = e && std::atoi(e) >= 2;
+  }
+  void operator()(const char* what) {
+    if (!on) return;
+    const auto n = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[spg] layout %-28s %.3f s\n", what, std::chrono::duration<double>(n - t).count());
+    t = n;
+  }
+};
+}  // namespace
+
 Layout build_layout(const ProblemDesc& p) {
+  PhaseTrace trace;
   Layout out;
   if (p.nblocks <= 0) throw LayoutError("no density blocks", -1);
   if (!(p.dx > 0)) throw LayoutError("dx must be positive", -1);
@@ -321,12 +367,20 @@ Layout build_layout(const ProblemDesc& p) {
       }
     for (int l = 0; l < out.nlev; ++l) H[l].map.set_box(box[l].data(), box[l].data() + 3);
   }
-  std::unordered_set<uint64_t> pages;
+  // a page holds two blocks: room for twice the input blocks
+  H0.bc.reserve((size_t)p.nblocks * 6);
+  H0.alloc.reserve((size_t)p.nblocks * 2);
+  H0.nflag.reserve((size_t)p.nblocks * 128);
+  H0.cflag.reserve((size_t)p.nblocks * 128);
+  std::vector<uint64_t> pages((size_t)p.nblocks);
   for (int64_t b = 0; b < p.nblocks; ++b) {
     const int32_t* bb = p.block_base + 3 * b;
     H0.mark_page(bb[0], bb[1], bb[2]);
-    pages.insert(bkey(bb[0] >> 2, bb[1] >> 2, bb[2] >> 3));
+    pages[(size_t)b] = bkey(bb[0] >> 2, bb[1] >> 2, bb[2] >> 3);
   }
+  std::sort(pages.begin(), pages.end());
+  pages.erase(std::unique(pages.begin(), pages.end()), pages.end());
+  trace("level-0 pages");
   out.n_allocated_pages = (int64_t)pages.size();
   // padding check (Multigrid_Hierarchy.h:145-157)
   for (uint64_t k : pages) {
@@ -340,14 +394,19 @@ Layout build_layout(const ProblemDesc& p) {
                           -2);
     }
   }
+  trace("padding check");
   // cell moduli (SIMP) and ACTIVE_CELL (Flaging.h:93-106)
-  std::vector<double> E0(H0.alloc.size() * 64, 0.0), R0(H0.alloc.size() * 64, 0.0);
+  // moduli per input cell (0 where the density is not positive or the cell
+  // lies outside the set pages); the level-0 vectors take them at the end
+  std::vector<double> Ein((size_t)p.nblocks * 64);
   parallel_ranges(p.nblocks, [&](int64_t b0, int64_t b1) {
     for (int64_t b = b0; b < b1; ++b) {
       const int32_t* bb = p.block_base + 3 * b;
       // a 4-aligned input block (the caller's layout) lies in one level-0 block
       const bool aligned = ((bb[0] | bb[1] | bb[2]) & 3) == 0;
       const int32_t hb0 = aligned ? H0.find(bb[0] >> 2, bb[1] >> 2, bb[2] >> 2) : -1;
+      double* eb = Ein.data() + b * 64;
+      std::fill(eb, eb + 64, 0.0);
       if (aligned && (hb0 < 0 || !H0.alloc[hb0])) continue;
       for (int l = 0; l < 64; ++l) {
         int x = bb[0] + (l >> 4), y = bb[1] + ((l >> 2) & 3), z = bb[2] + (l & 3);
@@ -355,16 +414,15 @@ Layout build_layout(const ProblemDesc& p) {
         if (hb < 0 || !H0.alloc[hb]) continue;  // cells outside set pages are dropped
         double d = p.density[b * 64 + l];
         if (std::isnan(d)) throw LayoutError("density contains NaN", -6);
-        size_t s = (size_t)hb * 64 + loc(x, y, z);
-        R0[s] = d;
         if (d > 0) {
           double e = stiffness_mapping(p.penalty, p.min_stiffness, p.min_density, d);
-          E0[s] = e;
-          if (e * p.mu != 0.0) H0.cflag[s] |= F_ACTIVE_CELL;
+          eb[l] = e;
+          if (e * p.mu != 0.0) H0.cflag[(size_t)hb * 64 + loc(x, y, z)] |= F_ACTIVE_CELL;
         }
       }
     }
   });
+  trace("cell moduli");
   // block activation check (Validate_Blocks.h:46-69); a cell's nodes lie in
   // its own block or its + neighbours, whose allocation is looked up once
   parallel_ranges((int64_t)H0.alloc.size(), [&](int64_t b0, int64_t b1) {
@@ -385,6 +443,7 @@ Layout build_layout(const ProblemDesc& p) {
     }
   }
   });
+  trace("activation check");
   // Dirichlet nodes -> Dirichlet cells (SPGrid_CPU_Solver.cpp:147-192)
   {
     std::unordered_set<uint64_t> dset[3];
@@ -405,38 +464,71 @@ Layout build_layout(const ProblemDesc& p) {
       H0.cflag[(size_t)hb * 64 + loc(x, y, z)] |= (uint16_t)(F_DIR_CELL_X << v);
     }
   }
+  trace("Dirichlet cells");
   // ---- coarse levels (Multigrid_Hierarchy.h:158-195)
   for (int l = 1; l < out.nlev; ++l) {
     HostLevel& F = H[l - 1];
     HostLevel& C = H[l];
-    size_t nbf = F.alloc.size();
+    // Fine cell x maps to coarse cell (x + B) >> 1: the cells of one fine
+    // block land in a 2x2x2 group of cells of a single coarse block.  The
+    // coarse blocks are created first, then every coarse cell gathers the
+    // ACTIVE / DIRICHLET bits of its 8 fine cells x = 2X - B + {0, 1}.
+    const size_t nbf = F.alloc.size();
     for (size_t b = 0; b < nbf; ++b) {
       if (!F.alloc[b]) continue;
-      for (int q = 0; q < 64; ++q) {
-        uint16_t cf = F.cflag[b * 64 + q];
-        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
-        int x = F.bc[3 * b] * 4 + (q >> 4), y = F.bc[3 * b + 1] * 4 + ((q >> 2) & 3), z = F.bc[3 * b + 2] * 4 + (q & 3);
-        int X = (x + kB[0]) >> 1, Y = (y + kB[1]) >> 1, Z = (z + kB[2]) >> 1;
-        int32_t cb = C.get(X >> 2, Y >> 2, Z >> 2);
-        C.cflag[(size_t)cb * 64 + loc(X, Y, Z)] |= (uint16_t)(cf & (F_ACTIVE_CELL | F_DIR_CELL));
-      }
+      bool any = false;
+      for (int q = 0; q < 64 && !any; ++q) any = (F.cflag[b * 64 + q] & (F_ACTIVE_CELL | F_DIR_CELL)) != 0;
+      if (!any) continue;
+      int cb[3];
+      for (int d = 0; d < 3; ++d) cb[d] = ((F.bc[3 * b + d] * 4 + kB[d]) >> 1) >> 2;
+      C.get(cb[0], cb[1], cb[2]);
     }
-    // pages of the coarse cells' nodes: all 8 for an ACTIVE cell, 1..7 for a
-    // DIRICHLET one; marked once per coarse cell (the union over the fine
-    // cells that map to it is the same set)
-    const size_t nbc = C.alloc.size();
-    for (size_t b = 0; b < nbc; ++b)
-      for (int q = 0; q < 64; ++q) {
-        const uint16_t cf = C.cflag[b * 64 + q];
-        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
-        const int X = C.bc[3 * b] * 4 + (q >> 4), Y = C.bc[3 * b + 1] * 4 + ((q >> 2) & 3), Z = C.bc[3 * b + 2] * 4 + (q & 3);
-        for (int c = (cf & F_ACTIVE_CELL) ? 0 : 1; c < 8; ++c)
-          C.mark_page(X + ((c >> 2) & 1), Y + ((c >> 1) & 1), Z + (c & 1));
+    const int64_t nbc = (int64_t)C.alloc.size();
+    parallel_ranges(nbc, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b) {
+        int f0[3];  // first fine block under coarse block b: fine x from 2 * 4 cb - B
+        for (int d = 0; d < 3; ++d) f0[d] = (2 * 4 * C.bc[3 * b + d] - kB[d]) >> 2;
+        int32_t fb[8];
+        for (int o = 0; o < 8; ++o) {
+          const int32_t t = F.find(f0[0] + ((o >> 2) & 1), f0[1] + ((o >> 1) & 1), f0[2] + (o & 1));
+          fb[o] = (t >= 0 && F.alloc[t]) ? t : -1;
+        }
+        for (int q = 0; q < 64; ++q) {
+          const int X = C.bc[3 * b] * 4 + (q >> 4), Y = C.bc[3 * b + 1] * 4 + ((q >> 2) & 3), Z = C.bc[3 * b + 2] * 4 + (q & 3);
+          uint16_t acc = 0;
+          for (int c = 0; c < 8; ++c) {
+            const int x = 2 * X - kB[0] + ((c >> 2) & 1), y = 2 * Y - kB[1] + ((c >> 1) & 1), z = 2 * Z - kB[2] + (c & 1);
+            const int o = (((x >> 2) - f0[0]) << 2) | (((y >> 2) - f0[1]) << 1) | ((z >> 2) - f0[2]);
+            if (fb[o] >= 0) acc |= F.cflag[(size_t)fb[o] * 64 + loc(x, y, z)];
+          }
+          C.cflag[(size_t)b * 64 + q] |= (uint16_t)(acc & (F_ACTIVE_CELL | F_DIR_CELL));
+        }
       }
+    });
+    // pages of the coarse cells' nodes: all 8 for an ACTIVE cell, 1..7 for a
+    // DIRICHLET one.  A cell's corner c lies in the block shifted by c & m,
+    // m = the axes on whose upper face the cell sits; per block the reached
+    // shifts are collected, then their pages are marked.
+    for (int64_t b = 0; b < nbc; ++b) {
+      unsigned reach = 0;
+      for (int q = 0; q < 64; ++q) {
+        const uint16_t cf = C.cflag[(size_t)b * 64 + q];
+        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+        const unsigned m = (((q >> 4) == 3) << 2) | ((((q >> 2) & 3) == 3) << 1) | ((q & 3) == 3);
+        for (unsigned c = (cf & F_ACTIVE_CELL) ? 0 : 1; c < 8; ++c) reach |= 1u << (c & m);
+      }
+      for (unsigned o = 0; o < 8; ++o)
+        if (reach & (1u << o))
+          C.mark_page((C.bc[3 * b] + ((o >> 2) & 1)) * 4, (C.bc[3 * b + 1] + ((o >> 1) & 1)) * 4,
+                      (C.bc[3 * b + 2] + (o & 1)) * 4);
+    }
   }
+  trace("coarse levels");
   for (int l = 0; l < out.nlev; ++l) flag_nodes(H[l]);
+  trace("node flags");
   for (int l = 0; l < out.nlev; ++l) flag_boundary(H[l]);
 
+  trace("boundary flags");
   // ---- GPU block sets: allocated blocks with an ACTIVE node, Morton order
   std::vector<BlockMap> gmap(out.nlev);
   for (int l = 0; l < out.nlev; ++l) gmap[l].set_box(box[l].data(), box[l].data() + 3);
@@ -462,21 +554,44 @@ Layout build_layout(const ProblemDesc& p) {
       int32_t b = sel[g].second;
       for (int d = 0; d < 3; ++d) D.base[3 * g + d] = Hl.bc[3 * b + d] * 4;
       gmap[l].insert(Hl.bc[3 * b], Hl.bc[3 * b + 1], Hl.bc[3 * b + 2], (int32_t)g);
-      for (int q = 0; q < 64; ++q) {
-        uint16_t f = Hl.nflag[(size_t)b * 64 + q];
-        uint8_t df = 0;
-        if (f & F_ACTIVE_NODE) { df |= D_ACTIVE; D.n_active++; }
-        if (f & 0x20) df |= 0x02;
-        if (f & 0x40) df |= 0x04;
-        if (f & 0x80) df |= 0x08;
-        if (f & F_BOUNDARY) df |= D_BOUNDARY;
-        D.flags[g * 64 + q] = df;
-        if ((df & D_BOUNDARY)) {
-          int x = D.base[3 * g] + (q >> 4), y = D.base[3 * g + 1] + ((q >> 2) & 3), z = D.base[3 * g + 2] + (q & 3);
-          int colour = ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
-          D.bnd_list[colour].push_back((int32_t)(g * 64 + q));
+    }
+    // device flags, boundary and Dirichlet lists: per range of blocks, the
+    // ranges' lists joined in block order
+    {
+      struct Part {
+        std::vector<int32_t> bnd[8], dir;
+        int64_t act = 0;
+      };
+      std::vector<Part> parts(64);
+      const int64_t nper = (D.nb + 63) / 64;
+      parallel_ranges(64, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r) {
+          Part& P = parts[(size_t)r];
+          const int64_t g1 = std::min(D.nb, (r + 1) * nper);
+          for (int64_t g = r * nper; g < g1; ++g) {
+            const int32_t b = sel[(size_t)g].second;
+            for (int q = 0; q < 64; ++q) {
+              const uint16_t f = Hl.nflag[(size_t)b * 64 + q];
+              uint8_t df = 0;
+              if (f & F_ACTIVE_NODE) { df |= D_ACTIVE; P.act++; }
+              if (f & 0x20) df |= 0x02;
+              if (f & 0x40) df |= 0x04;
+              if (f & 0x80) df |= 0x08;
+              if (f & F_BOUNDARY) df |= D_BOUNDARY;
+              D.flags[g * 64 + q] = df;
+              if ((df & D_BOUNDARY)) {
+                const int x = D.base[3 * g] + (q >> 4), y = D.base[3 * g + 1] + ((q >> 2) & 3), z = D.base[3 * g + 2] + (q & 3);
+                P.bnd[((x & 1) << 2) | ((y & 1) << 1) | (z & 1)].push_back((int32_t)(g * 64 + q));
+              }
+              if (df & D_DIR) P.dir.push_back((int32_t)(g * 64 + q));
+            }
+          }
         }
-        if (df & D_DIR) D.dir_list.push_back((int32_t)(g * 64 + q));
+      }, 1);
+      for (const Part& P : parts) {
+        D.n_active += P.act;
+        for (int c = 0; c < 8; ++c) D.bnd_list[c].insert(D.bnd_list[c].end(), P.bnd[c].begin(), P.bnd[c].end());
+        D.dir_list.insert(D.dir_list.end(), P.dir.begin(), P.dir.end());
       }
     }
     if (D.nb == 0) throw LayoutError("level " + std::to_string(l) + " has no active nodes", -1);
@@ -488,11 +603,14 @@ Layout build_layout(const ProblemDesc& p) {
                         -1);
     auto gfind = [&](int bx, int by, int bz) { return gmap[l].find(bx, by, bz); };
     D.nbr.resize(D.nb * 27);
-    for (int64_t g = 0; g < D.nb; ++g) {
-      int bx = D.base[3 * g] >> 2, by = D.base[3 * g + 1] >> 2, bz = D.base[3 * g + 2] >> 2;
-      for (int s = 0; s < 27; ++s) D.nbr[g * 27 + s] = gfind(bx + s / 9 - 1, by + (s / 3) % 3 - 1, bz + s % 3 - 1);
-    }
+    parallel_ranges(D.nb, [&](int64_t g0, int64_t g1) {
+      for (int64_t g = g0; g < g1; ++g) {
+        int bx = D.base[3 * g] >> 2, by = D.base[3 * g + 1] >> 2, bz = D.base[3 * g + 2] >> 2;
+        for (int s = 0; s < 27; ++s) D.nbr[g * 27 + s] = gfind(bx + s / 9 - 1, by + (s / 3) % 3 - 1, bz + s % 3 - 1);
+      }
+    });
   }
+  trace("block sets, neighbours");
   // boundary neighbour tables
   for (int l = 0; l < out.nlev; ++l) {
     LevelDesc& D = out.lv[l];
@@ -543,6 +661,7 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
   }
+  trace("boundary tables");
   // transfers
   for (int l = 0; l < out.nlev; ++l) {
     LevelDesc& D = out.lv[l];
@@ -567,6 +686,7 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
   }
+  trace("transfer tables");
   // explicit levels (Multigrid_Hierarchy.h:215-226): cells and Galerkin tables
   int first_explicit = out.nlev == 1 ? 0 : out.explicit_level;
   if (out.use_mg)
@@ -596,23 +716,10 @@ Layout build_layout(const ProblemDesc& p) {
         }
       }
     }
+  trace("explicit-level cells");
   // ---- level-0 cell data and input mapping
   {
     LevelDesc& D = out.lv[0];
-    out.E.assign(D.nb * 64, 0.0);
-    out.rho.assign(D.nb * 64, 0.0);
-    std::vector<int64_t> ncell(D.nb, 0);
-    parallel_ranges(D.nb, [&](int64_t g0, int64_t g1) {
-      for (int64_t g = g0; g < g1; ++g) {
-        int32_t hb = H0.find(D.base[3 * g] >> 2, D.base[3 * g + 1] >> 2, D.base[3 * g + 2] >> 2);
-        for (int q = 0; q < 64; ++q) {
-          size_t s = (size_t)hb * 64 + q;
-          if (H0.cflag[s] & F_ACTIVE_CELL) { out.E[g * 64 + q] = E0[s]; ncell[g]++; }
-          out.rho[g * 64 + q] = R0[s];
-        }
-      }
-    });
-    for (int64_t c : ncell) out.n_active_cells += c;
     out.nin_blocks = p.nblocks;
     out.in_node_slot.resize(p.nblocks * 64);
     parallel_ranges(p.nblocks, [&](int64_t b0, int64_t b1) {
@@ -630,14 +737,45 @@ Layout build_layout(const ProblemDesc& p) {
         }
       }
     });
+    // level-0 moduli and densities through the input map: a slot holds the
+    // input cell at its min corner (every ACTIVE cell is in a GPU block);
+    // slots without an input cell stay 0.  Input nodes outside the GPU set
+    // are collected per range for u_D below.
+    out.E.assign(D.nb * 64, 0.0);
+    out.rho.assign(D.nb * 64, 0.0);
+    const int64_t nparts = std::min<int64_t>(64, std::max<int64_t>(1, p.nblocks / 1024));
+    std::vector<int64_t> ncell(nparts, 0);
+    std::vector<std::vector<int64_t>> off_nodes(nparts);
+    parallel_ranges(nparts, [&](int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1; ++r) {
+        const int64_t b1 = std::min(p.nblocks, (r + 1) * ((p.nblocks + nparts - 1) / nparts));
+        for (int64_t b = r * ((p.nblocks + nparts - 1) / nparts); b < b1; ++b)
+          for (int l = 0; l < 64; ++l) {
+            const int64_t i = b * 64 + l;
+            const int32_t sl = out.in_node_slot[i];
+            if (sl < 0) {
+              off_nodes[r].push_back(i);
+              continue;
+            }
+            const double d = p.density[i];
+            if (d > 0 && Ein[i] * p.mu != 0.0) {
+              out.E[sl] = Ein[i];
+              ncell[r]++;
+            }
+            out.rho[sl] = d;
+          }
+      }
+    }, 1);
+    for (int64_t c : ncell) out.n_active_cells += c;
     // u_D (SPGrid_CPU_Solver.cpp:241-246,281-285): written at every listed node
     std::unordered_map<uint64_t, std::vector<int64_t>> in_nodes_off;  // for nodes outside the GPU set
-    for (int64_t b = 0; b < p.nblocks; ++b)
-      for (int l = 0; l < 64; ++l)
-        if (out.in_node_slot[b * 64 + l] < 0) {
-          const int32_t* bb = p.block_base + 3 * b;
-          in_nodes_off[bkey(bb[0] + (l >> 4), bb[1] + ((l >> 2) & 3), bb[2] + (l & 3))].push_back(b * 64 + l);
-        }
+    for (const auto& part : off_nodes)
+      for (int64_t i : part) {
+        const int64_t b = i / 64;
+        const int l = (int)(i % 64);
+        const int32_t* bb = p.block_base + 3 * b;
+        in_nodes_off[bkey(bb[0] + (l >> 4), bb[1] + ((l >> 2) & 3), bb[2] + (l & 3))].push_back(i);
+      }
     for (int64_t i = 0; i < p.ndir; ++i) {
       int x = p.dir_coord[3 * i], y = p.dir_coord[3 * i + 1], z = p.dir_coord[3 * i + 2];
       const int32_t gb = gmap[0].find(x >> 2, y >> 2, z >> 2);
@@ -654,7 +792,10 @@ Layout build_layout(const ProblemDesc& p) {
       }
     }
   }
-  band_distances(out);
+  trace("level-0 data, input map");
+  // the band distances serve only the overlapped first sweep (SPG_OVERLAP)
+  if (const char* e = std::getenv("SPG_OVERLAP"); e && std::atoi(e) != 0) band_distances(out);
+  trace("band distances");
   return out;
 }
 

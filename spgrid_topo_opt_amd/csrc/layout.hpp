@@ -154,7 +154,8 @@ struct LevelDesc {
   // steps, capped at kGsDistCap) and, per sweep direction (0 forward, 1
   // backward) and colour step s, the slots of that step's colour within
   // distance s + 1 of the band: the only ones whose update can depend on the
-  // boundary phase that precedes the sweep
+  // boundary phase that precedes the sweep (built only when SPG_OVERLAP
+  // selects the overlapped first sweep)
   std::vector<uint8_t> gs_dist;
   std::vector<int32_t> gs_near[2][8];
   int64_t n_active = 0;

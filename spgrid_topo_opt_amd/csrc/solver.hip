@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "solver.hpp"
 
@@ -84,6 +85,7 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   // flags, hierarchy and the index-range check on the host, before any
   // device call or allocation
   lay_ = build_layout(pd);
+  const double t_layout = now_s();
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw SpgError(SPG_E_HIP, "no HIP device available");
   if (cfg.device < 0 || cfg.device >= ndev) throw SpgError(SPG_E_INVALID, "device ordinal out of range");
@@ -101,13 +103,19 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   element_tables(kmu, kla);
   for (int e = 0; e < 576; ++e) K0_[e] = cfg.mu * kmu[e] + cfg.lambda * kla[e];
   upload_k0(K0_, cfg.mu, cfg.lambda, st_);
+  const double t_init = now_s();
   upload_layout();
+  sync();
+  const double t_upload = now_s();
   build_operators();
   sync();
   size_t free1 = 0;
   HIPCHK(hipMemGetInfo(&free1, &total0));
   dev_bytes_ = (double)free0 - (double)free1;
   setup_s_ = now_s() - t0;
+  if (const char* e = std::getenv("SPG_SETUP_TRACE"); e && std::atoi(e) > 0)
+    std::fprintf(stderr, "[spg] setup %.3f s: host layout %.3f, device init %.3f, upload %.3f, operators %.3f\n",
+                 setup_s_, t_layout - t0, t_init - t_layout, t_upload - t_init, now_s() - t_upload);
 }
 
 Solver::~Solver() {
@@ -248,9 +256,43 @@ void Solver::upload_layout() {
   HIPCHK(hipHostMalloc((void**)&hscal_, 256));
 }
 
+// Galerkin position matrices of one depth for this solver's K0, on its
+// device: built and uploaded once per process and kept until it ends (the
+// deepest usual one, depth 5, is 576 x 8^5 doubles = 151 MB), so solvers
+// rebuilt for new block sets or densities with the same material reuse them
+const double* Solver::posmat_device(int depth) {
+  struct Entry {
+    int dev, depth;
+    std::vector<double> k0;
+    double* ptr;
+  };
+  static std::mutex mu;
+  static std::vector<Entry>* cache = new std::vector<Entry>();  // never freed: device memory held to the end
+  const std::vector<double> k0(K0_, K0_ + 576);
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Entry& e : *cache)
+    if (e.dev == cfg_.device && e.depth == depth && e.k0 == k0) return e.ptr;
+  const std::vector<double> M = galerkin_position_matrices(depth, K0_);
+  double* p = nullptr;
+  HIPCHK(hipMalloc((void**)&p, M.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(p, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+  cache->push_back(Entry{cfg_.device, depth, k0, p});
+  return p;
+}
+
 void Solver::build_operators() {
   const int L = lay_.nlev;
   if (!lay_.use_mg) return;
+  const char* tr = std::getenv("SPG_SETUP_TRACE");
+  const bool trace = tr && std::atoi(tr) >= 2;
+  double tm = now_s();
+  auto mark = [&](const char* what) {
+    if (!trace) return;
+    sync();
+    const double t = now_s();
+    std::fprintf(stderr, "[spg] operators %-22s %.3f s\n", what, t - tm);
+    tm = t;
+  };
   const double one = 1.0, zero = 0.0;
   for (int l = 0; l < L; ++l) {
     const LevelDesc& D = lay_.lv[l];
@@ -261,10 +303,8 @@ void Solver::build_operators() {
     upload(R.cell_row, D.cell_row);
     if (!D.cell_fine_blocks.empty()) upload(R.cfb, D.cell_fine_blocks);
     upload(R.cgi, D.cell_gpu_index);
-    if ((int)posmat_cache_.size() <= depth) posmat_cache_.resize(depth + 1);
-    if (posmat_cache_[depth].empty()) posmat_cache_[depth] = galerkin_position_matrices(depth, K0_);
-    DBuf dM, G, Cm;
-    upload(dM, posmat_cache_[depth]);
+    const double* dM = posmat_device(depth);
+    DBuf G, Cm;
     const int64_t ncell = D.ncell;
     R.st64.alloc((size_t)D.nb * 243 * 64 * 8);
     R.st32.alloc((size_t)D.nb * 243 * 64 * 4);
@@ -274,14 +314,18 @@ void Solver::build_operators() {
       launch_galerkin_gather(st_, ncell, depth, R.cfb.as<int>(), R.cgi.as<int>(), E64_.as<double>(), G.as<double>());
       // Cm^T (576 x ncell) = M^T (576 x npos) * G^T (npos x ncell), column-major views
       RBCHK(rocblas_dgemm(rb_, rocblas_operation_none, rocblas_operation_none, 576, (rocblas_int)ncell,
-                          (rocblas_int)npos, &one, dM.as<double>(), 576, G.as<double>(), (rocblas_int)npos, &zero,
+                          (rocblas_int)npos, &one, dM, 576, G.as<double>(), (rocblas_int)npos, &zero,
                           Cm.as<double>(), 576));
     }
+    mark("  gather + product");
     launch_galerkin_assemble(st_, R.d, R.cell_row.as<int>(), Cm.as<double>(), cfg_.dx, R.st64.as<double>());
     launch_stencil_to_float(st_, R.st64.as<double>(), R.st32.as<float>(), (int64_t)D.nb * 243 * 64);
     sync();
+    mark("  assemble");
   }
+  mark("Galerkin levels");
   build_boundary_rows();
+  mark("boundary rows");
   // bottom: FP64 Cholesky of the Dirichlet-deleted stencil matrix, then its inverse
   // (Multigrid_Hierarchy.h:332-341; Elasticity_Matrix.cpp:15-233 factorizes with PARDISO)
   const int B = L - 1;
@@ -309,6 +353,7 @@ void Solver::build_operators() {
         nd_->upload(st_);
       }
       nd_->factor(lv_[B].st64.as<double>(), st_, rb_);
+      mark("bottom (nested dissection)");
     } catch (const NdNotSpd& e) {
       throw SpgError(SPG_E_SINGULAR, e.what());
     } catch (const SpgError&) {
@@ -372,6 +417,7 @@ void Solver::build_operators() {
   HIPCHK(hipMemsetAsync(xb64_.p, 0, npad * 8, st_));
   HIPCHK(hipMemsetAsync(xb32_.p, 0, npad * 4, st_));
   sync();
+  mark("bottom (dense inverse)");
 }
 
 // coefficient rows of the boundary system of every smoothed level

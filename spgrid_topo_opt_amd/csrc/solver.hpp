@@ -152,7 +152,7 @@ class Solver {
   double* hscal_ = nullptr;
   double setup_s_ = 0.0;
   double dev_bytes_ = 0.0;
-  std::vector<std::vector<double>> posmat_cache_;
+  const double* posmat_device(int depth);
   std::unordered_map<uint64_t, int32_t> gmap0_;
   bool have_solution_ = false;  // u_ holds a converged (or at least finite) solution
   bool have_guess_ = false;     // u_ holds a guess from set_initial_guess_in3
