@@ -1,6 +1,8 @@
 """torch.autograd path: GPU solve forward, hand-written sensitivity / adjoint
 backward, checked against the oracle's sensitivities and central finite
-differences (h = 1e-4 rho, FP64 solves at 1e-12; tolerance 1e-4 relative)."""
+differences (small box: h = 1e-4 rho, FP64 solves at 1e-12; the full-size
+128x64x64 box: Richardson-extrapolated pairs; tolerance 1e-4 relative per
+cell)."""
 import numpy as np
 import pytest
 
@@ -141,17 +143,46 @@ def test_narrow_band_loop_follows_the_band_rules():
     opt.close()
 
 
+def _richardson_fd(f, rho0, k, rel_step=1e-2):
+    """central differences of f at rho0[k] with steps h and 2h (h = rel_step
+    * rho), combined as (4 D(h) - D(2h)) / 3: truncation O(h^4), while the
+    compliance change stays far above the solves' rounding"""
+    h = rel_step * rho0[k].item()
+    d = []
+    for step in (h, 2 * h):
+        vals = []
+        for sgn in (1, -1):
+            r = rho0.clone()
+            r[k] += sgn * step
+            vals.append(f(r))
+        d.append((vals[0] - vals[1]) / (2 * step))
+    return (4 * d[0] - d[1]) / 3
+
+
 def test_cfg3_box128_autograd_check():
-    """The 128x64x64 cantilever at full size: the mixed-precision autograd
-    gradient of the compliance against the oracle's sensitivity field and
-    against central finite differences (h = 1e-4 rho, FP64 solves to 1e-12)
-    on 16 seeded cells; the adjoint-based gradient of the load-node
-    displacement against finite differences on 6 cells.  Tolerance 1e-4."""
+    """The 128x64x64 cantilever at full size, per sampled cell:
+    - the mixed-precision autograd gradient of the compliance against the
+      oracle's sensitivity field (FP64 solve to 1e-12);
+    - the compliance gradient (FP64 and mixed solves to 1e-12) against
+      central finite differences on 64 seeded cells;
+    - the adjoint-based gradient of the load-node displacement against
+      finite differences on 16 cells.
+    One cell's sensitivity is as small as 3e-9 of the compliance here, so a
+    1e-4 rho step moves the compliance by about 1e-12 relative, at the
+    rounding of the solves.  The finite differences are therefore
+    Richardson-extrapolated pairs (h = 1e-2 rho and 2h: truncation O(h^4))
+    of FP64 solves to 3e-14, and the gradients come from solves to 3e-14
+    too (measured: a gradient from a solve to 1e-12 is off by up to 1.5e-4
+    on the smallest sensitivities, the finite differences of solves to
+    1e-13 by up to 4e-4).  The compliance of these solves is reproducible
+    to about 1e-14 relative, which resolves a difference quotient to 1e-4
+    where rho |dc/drho| / c >= 3e-8; those cells (most) are checked to 1e-4
+    relative each, the ones below that floor to the floor's absolute size."""
     import time
     from spgrid_topo_opt_amd import topo_opt as T
     p = W.config("cfg3")
     rho0 = torch.tensor(p.density.reshape(-1), dtype=torch.float64, device="cuda")
-    femm = T.FEM(p, tolerance=1e-10, precision=0)
+    femm = T.FEM(p, tolerance=3e-14, precision=0)
     rho = rho0.clone().requires_grad_(True)
     t0 = time.perf_counter()
     c = T.compliance(rho, femm)
@@ -172,48 +203,45 @@ def test_cfg3_box128_autograd_check():
           flush=True)
     assert abs(c.item() - obj_o) / obj_o < 1e-8
     assert err_field < 1e-6
-    # At this size one cell's 1e-4 rho step moves the compliance by ~1e-10
-    # relative, so the finite differences of cells far from the load path are
-    # noise-limited: errors are measured against the largest sampled
-    # sensitivity (per-cell relative errors are printed).
-    fem64 = T.FEM(p, tolerance=1e-12, precision=1, max_iterations=2000)
-    cells = _cells(p, 16, seed=1)
-    fds = []
-    for k in cells:
-        h = 1e-4 * rho0[k].item()
-        vals = []
-        for sgn in (1, -1):
-            r = rho0.clone()
-            r[k] += sgn * h
-            vals.append(T.compliance(r, fem64).item())
-        fds.append((vals[0] - vals[1]) / (2 * h))
-    fds = np.array(fds)
-    scale = np.abs(fds).max()
-    err = np.abs(g[cells] - fds) / scale
-    print("cfg3 compliance FD: per-cell |g-fd|/|fd|", np.array2string(np.abs(g[cells] - fds) / np.abs(fds), precision=1),
-          f"|fd|/max {np.array2string(np.abs(fds) / scale, precision=1)} worst vs max {err.max():.2e}", flush=True)
-    assert err.max() <= 1e-4
+    # the FP64 gradient at the same densities
+    fem64 = T.FEM(p, tolerance=3e-14, precision=1, max_iterations=8000)
+    rho = rho0.clone().requires_grad_(True)
+    T.compliance(rho, fem64).backward()
+    g64 = rho.grad.cpu().numpy()
+    fd = T.FEM(p, tolerance=3e-14, precision=1, max_iterations=8000)
+    t0 = time.perf_counter()
+    cells = _cells(p, 64, seed=1)
+    fds = np.array([_richardson_fd(lambda r: T.compliance(r, fd).item(), rho0, k) for k in cells])
+    c0 = c.item()
+    rel_sens = rho0[cells].cpu().numpy() * np.abs(fds) / c0
+    ok = rel_sens >= 3e-8
+    floor = 1e-4 * 3e-8 * c0 / rho0[cells].cpu().numpy()  # 1e-4 of the floor's dc/drho, per cell
+    e64 = np.abs(g64[cells] - fds) / np.abs(fds)
+    emx = np.abs(g[cells] - fds) / np.abs(fds)
+    print(f"cfg3 compliance FD ({time.perf_counter() - t0:.0f}s): {ok.sum()} of {len(cells)} cells above the floor; "
+          f"per-cell |g-fd|/|fd| there: FP64 worst {e64[ok].max():.2e} median {np.median(e64[ok]):.1e}, mixed worst "
+          f"{emx[ok].max():.2e}; below: worst {e64[~ok].max() if (~ok).any() else 0:.2e}; rho|fd|/c from "
+          f"{rel_sens.min():.1e} to {rel_sens.max():.1e}", flush=True)
+    assert ok.sum() >= 48
+    assert e64[ok].max() <= 1e-4 and emx[ok].max() <= 1e-4
+    assert (np.abs(g64[cells] - fds)[~ok] <= floor[~ok]).all() and (np.abs(g[cells] - fds)[~ok] <= floor[~ok]).all()
     load = np.nonzero((nodes == p.force_coord[0]).all(1))[0][0]
     rho = rho0.clone().requires_grad_(True)
     j = T.displacement(rho, femm)[3 * load + 1]
     j.backward()
     gj = rho.grad.cpu().numpy()
-    cells = _cells(p, 6, seed=2)
-    fds = []
-    for k in cells:
-        h = 1e-4 * rho0[k].item()
-        vals = []
-        for sgn in (1, -1):
-            r = rho0.clone()
-            r[k] += sgn * h
-            vals.append(T.displacement(r, fem64)[3 * load + 1].item())
-        fds.append((vals[0] - vals[1]) / (2 * h))
-    fds = np.array(fds)
-    err = np.abs(gj[cells] - fds) / np.abs(fds).max()
-    print(f"cfg3 load-displacement adjoint FD worst vs max {err.max():.2e}", flush=True)
-    assert err.max() <= 1e-4
+    cells = _cells(p, 16, seed=2)
+    fds = np.array([_richardson_fd(lambda r: T.displacement(r, fd)[3 * load + 1].item(), rho0, k) for k in cells])
+    j0 = abs(j.item())
+    ok = rho0[cells].cpu().numpy() * np.abs(fds) / j0 >= 3e-8
+    err = np.abs(gj[cells] - fds) / np.abs(fds)
+    print(f"cfg3 load-displacement adjoint FD: {ok.sum()} of {len(cells)} cells above the floor, per-cell worst "
+          f"{err[ok].max():.2e}; |fd| from {np.abs(fds).min():.2e} to {np.abs(fds).max():.2e}", flush=True)
+    assert ok.sum() >= 10 and err[ok].max() <= 1e-4
+    assert (np.abs(gj[cells] - fds)[~ok] <= (3e-12 * j0 / rho0[cells].cpu().numpy())[~ok]).all()
     femm.close()
     fem64.close()
+    fd.close()
 
 
 @pytest.mark.parametrize("filter_size", [1.5, 1.0, 2.5])
