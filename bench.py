@@ -172,6 +172,7 @@ def main():
     log(f"[bench] vs {acc['reference']['source']}: mixed u err {acc['mixed']['u_rel_l2']:.2e}, "
         f"c err {acc['mixed']['c_rel']:.2e}; fp64 u err {acc['fp64']['u_rel_l2']:.2e}, c err {acc['fp64']['c_rel']:.2e}")
 
+    tr_gs, tr_ap = measured_traffic(GS_KERNEL, nn), measured_traffic(APPLY_KERNEL, nn)
     out = {
         "metric": METRIC,
         "value": t_mixed,
@@ -224,7 +225,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": GS_SWEEP_BYTES_PER_NODE_FP32 * nn / t_gs32 / 1e9 / HBM_PEAK_GBS,
-            "traffic": measured_traffic(GS_KERNEL, nn),
+            "traffic": tr_gs,
+            # the measured bytes at the measured time, against the same peak
+            "traffic_frac": None if tr_gs is None else tr_gs / (t_gs32 / 8) / 1e9 / HBM_PEAK_GBS,
             "us_per_launch": t_gs32 / 8 * 1e6,
             "algorithmic_bytes_per_launch": GS_SWEEP_BYTES_PER_NODE_FP32 / 8 * nn,
         },
@@ -235,7 +238,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": gbs32 / HBM_PEAK_GBS,
-            "traffic": measured_traffic(APPLY_KERNEL, nn),
+            "traffic": tr_ap,
+            "traffic_frac": None if tr_ap is None else tr_ap / t_ap32 / 1e9 / HBM_PEAK_GBS,
             "us_per_launch": t_ap32 * 1e6,
             "algorithmic_bytes_per_launch": BYTES_PER_NODE_FP32 * nn,
         },
