@@ -1985,7 +1985,10 @@ __global__ void __launch_bounds__(BC_THREADS) k_bnd0_coop(int n, BndOff L, Colou
     if (tid == 0) {
       int b = 0;
       const unsigned long long t0 = wall_clock64();
-      while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      // relaxed polls (an acquire load would invalidate the L2 at every
+      // poll), then one acquire fence for the data the other workgroups
+      // published with their release add
+      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (wall_clock64() - t0 > limit) {
           __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           b = 1;
@@ -1993,7 +1996,7 @@ __global__ void __launch_bounds__(BC_THREADS) k_bnd0_coop(int n, BndOff L, Colou
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       bad = b;
     }
     __syncthreads();
