@@ -109,6 +109,7 @@ int spg_create(const spg_config* cfg, int64_t nblocks, const int32_t* block_base
                spg_solver** out) {
   API_TRY
   if (!cfg || !out || (nblocks > 0 && (!block_base || !density))) throw SpgError(SPG_E_INVALID, "null argument");
+  if (nblocks < 0 || ndir < 0) throw SpgError(SPG_E_INVALID, "negative block or Dirichlet count");
   if (ndir > 0 && (!dir_coord || !dir_axis || !dir_value)) throw SpgError(SPG_E_INVALID, "null Dirichlet arrays");
   *out = nullptr;
   auto h = std::make_unique<spg_solver>();
@@ -145,6 +146,7 @@ int spg_update_density_device(spg_solver* s, const double* density_dev) {
 int spg_set_forces(spg_solver* s, int64_t n, const int32_t* coord, const double* value) {
   API_TRY
   if (!s || (n > 0 && (!coord || !value))) throw SpgError(SPG_E_INVALID, "null argument");
+  if (n < 0) throw SpgError(SPG_E_INVALID, "negative force count");
   s->s->set_forces(n, coord, value);
   return SPG_OK;
   API_CATCH

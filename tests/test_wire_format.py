@@ -200,6 +200,25 @@ def test_index_range_is_checked_before_any_device_call(monkeypatch):
         _lib.Solver(p)
 
 
+def test_negative_counts_are_refused_before_any_device_call():
+    """spg_create and spg_set_forces check their counts on the host: a
+    negative block, Dirichlet or force count returns SPG_E_INVALID"""
+    import ctypes as C
+    from spgrid_topo_opt_amd import workloads as W
+    p = W.box(8, 4, 4)
+    cfg = _lib.make_config(p)
+    L = _lib.lib()
+    bb = np.ascontiguousarray(p.block_base, np.int32)
+    de = np.ascontiguousarray(p.density, np.float64)
+    h = C.c_void_p()
+    ip, dp = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    for nb, nd in ((-1, 0), (bb.shape[0], -3)):
+        rc = L.spg_create(C.byref(cfg), nb, bb.ctypes.data_as(ip), de.ctypes.data_as(dp), nd, None, None, None,
+                          C.byref(h))
+        assert rc == _lib.SPG_E_INVALID and "negative" in L.spg_last_error().decode()
+        assert not h.value
+
+
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "spgrid_topo_opt_amd.h")).read()
     import re
