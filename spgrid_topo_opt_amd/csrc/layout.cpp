@@ -153,19 +153,31 @@ static void parallel_ranges(int64_t n, F&& f, int64_t min_chunk = 4096) {
 // are then gathered node by node, so blocks are independent.
 static void flag_nodes(HostLevel& H) {
   const size_t nb0 = H.alloc.size();
-  for (size_t b = 0; b < nb0; ++b) {
-    if (!H.alloc[b]) continue;
-    unsigned upper = 0;  // axes (x 4, y 2, z 1) on whose upper face a flagged cell sits
-    bool any = false;
-    for (int l = 0; l < 64; ++l) {
-      if (!(H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
-      any = true;
-      upper |= (((l >> 4) == 3) << 2) | ((((l >> 2) & 3) == 3) << 1) | ((l & 3) == 3);
+  // per block, the + neighbours (bit o) it reaches that do not exist yet;
+  // they are then created in block order, as a serial pass would
+  std::vector<uint8_t> missing(nb0, 0);
+  parallel_ranges((int64_t)nb0, [&](int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; ++b) {
+      if (!H.alloc[b]) continue;
+      unsigned upper = 0;  // axes (x 4, y 2, z 1) on whose upper face a flagged cell sits
+      bool any = false;
+      for (int l = 0; l < 64; ++l) {
+        if (!(H.cflag[b * 64 + l] & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+        any = true;
+        upper |= (((l >> 4) == 3) << 2) | ((((l >> 2) & 3) == 3) << 1) | ((l & 3) == 3);
+      }
+      if (!any) continue;
+      const int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
+      for (unsigned o = 1; o < 8; ++o)
+        if ((o & upper) == o && H.find(bx + ((o >> 2) & 1), by + ((o >> 1) & 1), bz + (o & 1)) < 0)
+          missing[b] |= (uint8_t)(1u << o);
     }
-    if (!any) continue;
+  });
+  for (size_t b = 0; b < nb0; ++b) {
+    if (!missing[b]) continue;
     const int bx = H.bc[3 * b], by = H.bc[3 * b + 1], bz = H.bc[3 * b + 2];
     for (unsigned o = 1; o < 8; ++o)
-      if ((o & upper) == o) H.get(bx + ((o >> 2) & 1), by + ((o >> 1) & 1), bz + (o & 1));
+      if (missing[b] & (1u << o)) H.get(bx + ((o >> 2) & 1), by + ((o >> 1) & 1), bz + (o & 1));
   }
   const int64_t nb1 = (int64_t)H.alloc.size();
   parallel_ranges(nb1, [&](int64_t b0, int64_t b1) {
@@ -372,33 +384,43 @@ Layout build_layout(const ProblemDesc& p) {
   H0.alloc.reserve((size_t)p.nblocks * 2);
   H0.nflag.reserve((size_t)p.nblocks * 128);
   H0.cflag.reserve((size_t)p.nblocks * 128);
-  std::vector<uint64_t> pages((size_t)p.nblocks);
   for (int64_t b = 0; b < p.nblocks; ++b) {
     const int32_t* bb = p.block_base + 3 * b;
     H0.mark_page(bb[0], bb[1], bb[2]);
-    pages[(size_t)b] = bkey(bb[0] >> 2, bb[1] >> 2, bb[2] >> 3);
   }
-  std::sort(pages.begin(), pages.end());
-  pages.erase(std::unique(pages.begin(), pages.end()), pages.end());
   trace("level-0 pages");
-  out.n_allocated_pages = (int64_t)pages.size();
-  // padding check (Multigrid_Hierarchy.h:145-157)
-  for (uint64_t k : pages) {
-    int pc[3] = {(int)((k >> 42) & 0x1fffff) - (1 << 20), (int)((k >> 21) & 0x1fffff) - (1 << 20),
-                 (int)(k & 0x1fffff) - (1 << 20)};
-    for (int d = 0; d < 3; ++d) {
-      int o = pc[d] * kPage[d];
-      if (o < kB[d] || o >= p.res[d] - 2 * kB[d])
-        throw LayoutError("padding violated: page origin (" + std::to_string(pc[0] * 4) + "," +
-                              std::to_string(pc[1] * 4) + "," + std::to_string(pc[2] * 8) + ") outside [B, res-2B)",
-                          -2);
+  // every page marked so far holds two allocated blocks: its lower one (even
+  // z) stands for it
+  {
+    int64_t npages = 0;
+    bool bad = false;
+    uint64_t first_bad = ~0ull;  // the violation reported: the smallest page key, as in key order
+    for (size_t b = 0; b < H0.alloc.size(); ++b) {
+      if (!H0.alloc[b] || (H0.bc[3 * b + 2] & 1)) continue;
+      ++npages;
+      // padding check (Multigrid_Hierarchy.h:145-157)
+      const int o[3] = {H0.bc[3 * b] * 4, H0.bc[3 * b + 1] * 4, H0.bc[3 * b + 2] * 4};
+      for (int d = 0; d < 3; ++d)
+        if (o[d] < kB[d] || o[d] >= p.res[d] - 2 * kB[d]) {
+          bad = true;
+          first_bad = std::min(first_bad, bkey(o[0] >> 2, o[1] >> 2, o[2] >> 3));
+        }
+    }
+    out.n_allocated_pages = npages;
+    if (bad) {
+      const uint64_t k = first_bad;
+      const int pc[3] = {(int)((k >> 42) & 0x1fffff) - (1 << 20), (int)((k >> 21) & 0x1fffff) - (1 << 20),
+                         (int)(k & 0x1fffff) - (1 << 20)};
+      throw LayoutError("padding violated: page origin (" + std::to_string(pc[0] * 4) + "," +
+                            std::to_string(pc[1] * 4) + "," + std::to_string(pc[2] * 8) + ") outside [B, res-2B)",
+                        -2);
     }
   }
   trace("padding check");
   // cell moduli (SIMP) and ACTIVE_CELL (Flaging.h:93-106)
   // moduli per input cell (0 where the density is not positive or the cell
   // lies outside the set pages); the level-0 vectors take them at the end
-  std::vector<double> Ein((size_t)p.nblocks * 64);
+  NodeArray<double> Ein((size_t)p.nblocks * 64);
   parallel_ranges(p.nblocks, [&](int64_t b0, int64_t b1) {
     for (int64_t b = b0; b < b1; ++b) {
       const int32_t* bb = p.block_base + 3 * b;
@@ -474,11 +496,17 @@ Layout build_layout(const ProblemDesc& p) {
     // coarse blocks are created first, then every coarse cell gathers the
     // ACTIVE / DIRICHLET bits of its 8 fine cells x = 2X - B + {0, 1}.
     const size_t nbf = F.alloc.size();
+    std::vector<uint8_t> flagged(nbf, 0);
+    parallel_ranges((int64_t)nbf, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b) {
+        if (!F.alloc[b]) continue;
+        bool any = false;
+        for (int q = 0; q < 64 && !any; ++q) any = (F.cflag[b * 64 + q] & (F_ACTIVE_CELL | F_DIR_CELL)) != 0;
+        flagged[b] = any;
+      }
+    });
     for (size_t b = 0; b < nbf; ++b) {
-      if (!F.alloc[b]) continue;
-      bool any = false;
-      for (int q = 0; q < 64 && !any; ++q) any = (F.cflag[b * 64 + q] & (F_ACTIVE_CELL | F_DIR_CELL)) != 0;
-      if (!any) continue;
+      if (!flagged[b]) continue;
       int cb[3];
       for (int d = 0; d < 3; ++d) cb[d] = ((F.bc[3 * b + d] * 4 + kB[d]) >> 1) >> 2;
       C.get(cb[0], cb[1], cb[2]);
@@ -509,14 +537,21 @@ Layout build_layout(const ProblemDesc& p) {
     // DIRICHLET one.  A cell's corner c lies in the block shifted by c & m,
     // m = the axes on whose upper face the cell sits; per block the reached
     // shifts are collected, then their pages are marked.
-    for (int64_t b = 0; b < nbc; ++b) {
-      unsigned reach = 0;
-      for (int q = 0; q < 64; ++q) {
-        const uint16_t cf = C.cflag[(size_t)b * 64 + q];
-        if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
-        const unsigned m = (((q >> 4) == 3) << 2) | ((((q >> 2) & 3) == 3) << 1) | ((q & 3) == 3);
-        for (unsigned c = (cf & F_ACTIVE_CELL) ? 0 : 1; c < 8; ++c) reach |= 1u << (c & m);
+    std::vector<uint8_t> reached((size_t)nbc, 0);
+    parallel_ranges(nbc, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; ++b) {
+        unsigned reach = 0;
+        for (int q = 0; q < 64; ++q) {
+          const uint16_t cf = C.cflag[(size_t)b * 64 + q];
+          if (!(cf & (F_ACTIVE_CELL | F_DIR_CELL))) continue;
+          const unsigned m = (((q >> 4) == 3) << 2) | ((((q >> 2) & 3) == 3) << 1) | ((q & 3) == 3);
+          for (unsigned c = (cf & F_ACTIVE_CELL) ? 0 : 1; c < 8; ++c) reach |= 1u << (c & m);
+        }
+        reached[(size_t)b] = (uint8_t)reach;
       }
+    });
+    for (int64_t b = 0; b < nbc; ++b) {
+      const unsigned reach = reached[(size_t)b];
       for (unsigned o = 0; o < 8; ++o)
         if (reach & (1u << o))
           C.mark_page((C.bc[3 * b] + ((o >> 2) & 1)) * 4, (C.bc[3 * b + 1] + ((o >> 1) & 1)) * 4,
@@ -741,8 +776,12 @@ Layout build_layout(const ProblemDesc& p) {
     // input cell at its min corner (every ACTIVE cell is in a GPU block);
     // slots without an input cell stay 0.  Input nodes outside the GPU set
     // are collected per range for u_D below.
-    out.E.assign(D.nb * 64, 0.0);
-    out.rho.assign(D.nb * 64, 0.0);
+    out.E.resize(D.nb * 64);
+    out.rho.resize(D.nb * 64);
+    parallel_ranges(D.nb, [&](int64_t b0, int64_t b1) {
+      std::fill(out.E.begin() + b0 * 64, out.E.begin() + b1 * 64, 0.0);
+      std::fill(out.rho.begin() + b0 * 64, out.rho.begin() + b1 * 64, 0.0);
+    });
     const int64_t nparts = std::min<int64_t>(64, std::max<int64_t>(1, p.nblocks / 1024));
     std::vector<int64_t> ncell(nparts, 0);
     std::vector<std::vector<int64_t>> off_nodes(nparts);

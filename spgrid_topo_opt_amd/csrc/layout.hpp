@@ -12,9 +12,12 @@
 // other node carries zeros in all solver vectors.
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace spg {
@@ -167,17 +170,41 @@ struct LevelDesc {
   std::vector<int32_t> cell_gpu_index;    // [ncell] blk*64+l of the cell's min corner
 };
 
+// An allocator that leaves elements uninitialised on resize: the per-node
+// arrays of a level are hundreds of MB, and every element is written by a
+// parallel loop, so a serial zero fill on allocation would only cost time.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using NodeArray = std::vector<T, NoInitAlloc<T>>;
+
 struct Layout {
   int nlev = 0;
   bool use_mg = true;
   int explicit_level = 2;
   std::vector<LevelDesc> lv;
   // level-0 cell data (per node slot = cell at min corner)
-  std::vector<double> E;          // stiffness mapping of active cells, 0 otherwise
-  std::vector<double> rho;        // input density (0 if none)
+  NodeArray<double> E;            // stiffness mapping of active cells, 0 otherwise
+  NodeArray<double> rho;          // input density (0 if none)
   // input block nodes -> level-0 slot (blk*64+l) or -1
   int64_t nin_blocks = 0;
-  std::vector<int32_t> in_node_slot;   // [nin_blocks*64]
+  NodeArray<int32_t> in_node_slot;   // [nin_blocks*64]
   // Dirichlet values u_D at level-0 slots, and at input nodes outside the GPU set
   std::vector<int32_t> ud_slot;        // slot*3+axis
   std::vector<double> ud_value;

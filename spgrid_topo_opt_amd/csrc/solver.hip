@@ -61,8 +61,9 @@ void DBuf::alloc(size_t n) {
   if (n) HIPCHK(hipMalloc(&p, n));
 }
 
-template <class T>
-static void upload(DBuf& b, const std::vector<T>& v) {
+template <class V>
+static void upload(DBuf& b, const V& v) {
+  using T = typename V::value_type;
   b.alloc(v.size() * sizeof(T));
   if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
@@ -240,8 +241,8 @@ void Solver::upload_layout() {
   }
   // level-0 cell data
   upload(E64_, lay_.E);
-  std::vector<float> e32(lay_.E.begin(), lay_.E.end());
-  upload(E32_, e32);
+  E32_.alloc(lay_.E.size() * 4);  // rounded on the device (round to nearest, as a host cast)
+  launch_cast_d2f(st_, E64_.as<double>(), E32_.as<float>(), (int64_t)lay_.E.size());
   upload(rho_, lay_.rho);
   upload(in_slot_, lay_.in_node_slot);
   upload(ud_idx_, lay_.ud_slot);
@@ -501,8 +502,8 @@ void Solver::update_density(const double* density) {
         throw SpgError(SPG_E_INVALID, "density update would activate a cell (topology change)");
       }
     }
-  lay_.E = E;
-  lay_.rho = rho;
+  lay_.E.assign(E.begin(), E.end());
+  lay_.rho.assign(rho.begin(), rho.end());
   HIPCHK(hipMemcpy(E64_.p, E.data(), E.size() * 8, hipMemcpyHostToDevice));
   std::vector<float> e32(E.begin(), E.end());
   HIPCHK(hipMemcpy(E32_.p, e32.data(), e32.size() * 4, hipMemcpyHostToDevice));

@@ -219,6 +219,23 @@ def test_negative_counts_are_refused_before_any_device_call():
         assert not h.value
 
 
+@pytest.mark.parametrize("shift", [-8, -4, 4 * 60, 4 * 61])
+def test_padding_violation_names_the_oracles_page(shift):
+    """a block set too close to the grid edge is refused on the host
+    (Multigrid_Hierarchy.h:145-157) with the page the oracle names"""
+    import re
+    from oracle import oracle as O
+    from spgrid_topo_opt_amd import workloads as W
+    p = W.box(8, 4, 4)
+    r = W.Problem(**{**p.__dict__, "block_base": p.block_base + shift})
+    with pytest.raises(RuntimeError, match="padding") as eo:
+        O.Oracle(r)
+    with pytest.raises(_lib.SpgError, match="padding") as ep:
+        _lib.Solver(r)
+    page = re.compile(r"page origin \((-?\d+),(-?\d+),(-?\d+)\)")
+    assert page.search(str(eo.value)).groups() == page.search(str(ep.value)).groups()
+
+
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "spgrid_topo_opt_amd.h")).read()
     import re
