@@ -858,7 +858,9 @@ template <class T, int QB>
 __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, const int* __restrict__ nbr,
                                                          const uint8_t* __restrict__ flags, const T* __restrict__ E,
                                                          T dx, T* __restrict__ u, const T* __restrict__ b,
-                                                         const uint8_t* __restrict__ dist, int dmin) {
+                                                         const uint8_t* __restrict__ dist, int dmin,
+                                                         const T* __restrict__ bcm = nullptr,
+                                                         const uint8_t* __restrict__ fcm = nullptr) {
   static_assert(QB >= 4, "the 125-entry halo table is filled by one pass of the workgroup");
   __shared__ T hu[QB][3][128];
   __shared__ int ht[QB][27];
@@ -902,10 +904,18 @@ __global__ void __launch_bounds__(32 * QB) k_gs0_colour4(int nb, int colour, con
       ee[r] = eb >= 0 ? E[eb * 64 + loc3(ex, ey, ez)] : T(0);
     }
     if (q == 0 && blk < nb) {
-      fl = flags[blk * 64 + l];
-      if (dist) skip = dist[blk * 64 + l] < dmin;
+      if (bcm) {
+        // colour-major copies: the colour's 8 nodes of a block are adjacent
+        const int64_t cb = (int64_t)colour * nb + blk;
+        fl = fcm[cb * 8 + k];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) bv[c] = b[vidx(blk, c, l)];
+        for (int c = 0; c < 3; ++c) bv[c] = bcm[(cb * 3 + c) * 8 + k];
+      } else {
+        fl = flags[blk * 64 + l];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bv[c] = b[vidx(blk, c, l)];
+      }
+      if (dist) skip = dist[blk * 64 + l] < dmin;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -2902,12 +2912,37 @@ void launch_bnd_coef_st(hipStream_t st, const LevelDev& L, const double* ST, dou
   if (n > 0) k_bnd_coef_st<<<nblk((int64_t)n * 32), TPB, 0, st>>>(n, L.bnd, ST, A);
 }
 
+// colour-major copies: thread i = (blk, comp, l) of the node-major vector
+__device__ __forceinline__ int colour_of(int l) { return (((l >> 4) & 1) << 2) | (((l >> 2) & 1) << 1) | (l & 1); }
+__device__ __forceinline__ int colour_k(int l) { return (((l >> 5) & 1) << 2) | (((l >> 3) & 1) << 1) | ((l >> 1) & 1); }
+template <class T>
+__global__ void k_colour_major(int nb, const T* __restrict__ v, T* __restrict__ vcm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nb * 192) return;
+  const int blk = (int)(i / 192), comp = (int)((i / 64) % 3), l = (int)(i & 63);
+  vcm[(((int64_t)colour_of(l) * nb + blk) * 3 + comp) * 8 + colour_k(l)] = v[i];
+}
+__global__ void k_colour_major_flags(int nb, const uint8_t* __restrict__ flags, uint8_t* __restrict__ fcm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nb * 64) return;
+  const int blk = (int)(i >> 6), l = (int)(i & 63);
+  fcm[((int64_t)colour_of(l) * nb + blk) * 8 + colour_k(l)] = flags[i];
+}
+template <class T>
+void launch_colour_major(hipStream_t st, const LevelDev& L, const T* v, T* vcm) {
+  k_colour_major<T><<<nblk((int64_t)L.nb * 192), TPB, 0, st>>>(L.nb, v, vcm);
+}
+void launch_colour_major_flags(hipStream_t st, int nb, const uint8_t* flags, uint8_t* fcm) {
+  k_colour_major_flags<<<nblk((int64_t)nb * 64), TPB, 0, st>>>(nb, flags, fcm);
+}
+
 // SPG_GS_KERNEL selects the level-0 colour kernel: 1 the octet kernel
 // k_gs0_colour, 2 the corner-per-wave k_gs0_colour_uc, 3 the quad-lane
 // k_gs0_colour4; 0 or unset picks per precision.  All are parity-tested.
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour,
-                const uint8_t* dist, int dmin) {
+                const uint8_t* dist, int dmin, const T* bcm) {
+  if (!L.flags_cm) bcm = nullptr;
   if (dist) {
     // the far part of a sweep beside the band phase: the quad kernel
     k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, dist, dmin);
@@ -2938,15 +2973,16 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
     // sweep: 4 -> 284-308 / 497-509 us, 8 -> 279-281 / 468 us, 16 -> 300 / 489 us
     const char* q = std::getenv("SPG_GS_QB");
     const int qb = q ? std::atoi(q) : 8;
+    const uint8_t* fcm = bcm ? L.flags_cm : nullptr;
     if (qb == 4)
-      k_gs0_colour4<T, 4><<<(L.nb + 3) / 4, 128, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
-                                                                  0);
+      k_gs0_colour4<T, 4><<<(L.nb + 3) / 4, 128, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr, 0,
+                                                           bcm, fcm);
     else if (qb == 16)
-      k_gs0_colour4<T, 16><<<(L.nb + 15) / 16, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
-                                                                  0);
+      k_gs0_colour4<T, 16><<<(L.nb + 15) / 16, 512, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr, 0,
+                                                              bcm, fcm);
     else
-      k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr,
-                                                                  0);
+      k_gs0_colour4<T, 8><<<(L.nb + 7) / 8, 256, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b, nullptr, 0,
+                                                           bcm, fcm);
   } else {
     const int nwg = (L.nb + GS_WAVES - 1) / GS_WAVES;
     k_gs0_colour<T><<<nwg, 64 * GS_WAVES, 0, st>>>(L.nb, colour, L.nbr, L.flags, E, dx, u, b);
@@ -3183,7 +3219,9 @@ void launch_set_entries(hipStream_t st, double* v, const int* idx, const double*
   template void launch_apply<T>(hipStream_t, const LevelDev&, const T*, T, const T*, const T*, T*, int);         \
   template void launch_apply_dot<T>(hipStream_t, const LevelDev&, const T*, T, const T*, T*, double*, double*,   \
                                     double*, CgScalars*, int);                                                  \
-  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, const uint8_t*, int); \
+  template void launch_gs0<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, const uint8_t*, int,  \
+                              const T*);                                                                        \
+  template void launch_colour_major<T>(hipStream_t, const LevelDev&, const T*, T*);                              \
   template void launch_gs0_list<T>(hipStream_t, const LevelDev&, const int*, const int*, int, const T*, T, T*,      \
                                    const T*);                                                                       \
   template void launch_gs0_boundary<T>(hipStream_t, const LevelDev&, const T*, T, T*, const T*, int, bool);      \

@@ -52,6 +52,9 @@ struct LevelDev {
   int bmap_sweeps = 0;
   // level 0: slots of the 27 neighbours of every boundary node ([n*27], -1 absent)
   const int* bnslot = nullptr;
+  // level 0: the flags in colour-major order ([8][nb][8]: colour, block, k-th
+  // node of the colour in the block), read by the colour sweep
+  const uint8_t* flags_cm = nullptr;
 };
 
 // Colour of the i-th step of a Gauss-Seidel sweep: level 0 runs the colours
@@ -94,10 +97,17 @@ int64_t apply_dot_partials(int64_t nb);
 template <class T>
 void launch_apply_dot(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, T* out, double* bpart,
                       double* part, double* dot, CgScalars* sc = nullptr, int stage = -1);
-// level-0 GS: one colour over all blocks; all boundary sweeps of one phase
+// level-0 GS: one colour over all blocks; all boundary sweeps of one phase.
+// With `bcm` (b in colour-major order, launch_colour_major) and the level's
+// flags_cm, the colour kernel reads b and the flags of its colour only.
 template <class T>
 void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const T* b, int colour,
-                const uint8_t* dist = nullptr, int dmin = 0);
+                const uint8_t* dist = nullptr, int dmin = 0, const T* bcm = nullptr);
+// v in colour-major order: vcm[((c * nb + blk) * 3 + comp) * 8 + k] = v at the
+// k-th node of colour c in block blk (level-0 vectors)
+template <class T>
+void launch_colour_major(hipStream_t st, const LevelDev& L, const T* v, T* vcm);
+void launch_colour_major_flags(hipStream_t st, int nb, const uint8_t* flags, uint8_t* fcm);
 // one level-0 colour step over a list of slots (27 neighbour slots each in nslot)
 template <class T>
 void launch_gs0_list(hipStream_t st, const LevelDev& L, const int* list, const int* nslot, int n, const T* E, T dx,

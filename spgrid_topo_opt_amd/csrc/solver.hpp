@@ -101,7 +101,12 @@ class Solver {
   // z = M b0; `z_is_zero` says the caller has already cleared z
   template <class T> void vcycle(const T* b0, T* z, bool z_is_zero = false);
   void vcycle_coarse64(const float* b0, float* z, bool z_is_zero);
-  template <class T> void smooth(int l, T* u, const T* b, bool forward);
+  template <class T> void smooth(int l, T* u, const T* b, bool forward, const T* bcm = nullptr);
+  // the level-0 colour sweeps read b and the flags from colour-major copies
+  // when the level's working set exceeds the memory-side cache (or
+  // SPG_GS_CMB=1); returns the copy of b0 to pass, or null
+  template <class T> const T* colour_major_b(const T* b0);
+  int cmb_mode_ = -1;  // SPG_GS_CMB: 0 never, 1 always, unset by size
   template <class T> void level_residual(int l, const T* u, const T* b, T* r);
   template <class T> void bottom(const T* b, T* u);
   template <class T> int pcg(T* x, T* r, double tol, int maxit, int* iters, double* last_nrm);
@@ -149,6 +154,7 @@ class Solver {
   DBuf u_, f_, res_, tmp_;
   DBuf scal_, part_, pinned_;
   DBuf apart_;  // per-workgroup partials of the fused operator apply + dot
+  DBuf fcm_, bcm_;  // level-0 flags and b in colour-major order (either precision)
   double* hscal_ = nullptr;
   double setup_s_ = 0.0;
   double dev_bytes_ = 0.0;

@@ -218,6 +218,39 @@ def test_gs_colour_kernels(blocky, prec, tol, kind, monkeypatch):
         assert rel(got, want) < tol
 
 
+@pytest.mark.parametrize("prec,tol,vtol", [(1, 1e-11, 1e-10), (0, 5e-5, 1e-4)])
+def test_gs_with_colour_major_b(prec, tol, vtol, monkeypatch):
+    """SPG_GS_CMB=1: the level-0 colour sweeps read b and the flags from
+    colour-major copies (by default only above the memory-side cache size).
+    Sweeps in both directions and the V-cycle match the oracle, and solves
+    give the node-major reads' iterations and bits."""
+    monkeypatch.setenv("SPG_GS_CMB", "1")
+    P = Pair(W.random_blocky(seed=11, n=(16, 12, 12), mg_levels=4))
+    rng = np.random.default_rng(7)
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    u = rng.standard_normal((n, 3)) * 1e-6
+    u[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    b = rng.standard_normal((n, 3))
+    for fwd in (True, False):
+        got = P.g.kernel(_lib.OP_GS_SWEEP, 0, u, b, precision=prec, forward=fwd)
+        want = P.from_o(0, P.o.gs_sweep(0, P.to_o(0, u), P.to_o(0, b), fwd, False))
+        assert rel(got, want) < tol
+    b[(gf & 1) == 0] = 0
+    b[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    got = P.g.kernel(_lib.OP_VCYCLE, 0, None, b, precision=prec)
+    want = P.from_o(0, P.o.vcycle(P.to_o(0, b), O.MODE_FP64))
+    assert rel(got, want) < vtol
+    monkeypatch.setenv("SPG_GS_CMB", "0")
+    g0 = _lib.Solver(P.p)
+    kw = dict(tolerance=1e-10, precision=1, max_iterations=500) if prec == 1 else dict(tolerance=1e-7)
+    (o1, i1), (o0, i0) = P.g.solve(**kw) if prec == 1 else P.g.solve_mixed(**kw), \
+        g0.solve(**kw) if prec == 1 else g0.solve_mixed(**kw)
+    assert (i1["iterations"], i1["passes"]) == (i0["iterations"], i0["passes"])
+    assert np.array_equal(o1, o0)
+    g0.close()
+
+
 @pytest.mark.parametrize("host", [True, False])
 def test_gs_sweep_follows_a_density_update(host):
     """density updates (host and device) reach the level-0 colour sweeps and
