@@ -7,8 +7,8 @@
 - 2048x1024x1024 ellipsoidal shell (281 M DOF): the mixed solve, checked
   through size-independent identities (no oracle solve fits the test's time).
 
-Tolerances are written in each test.  The north star's bar is 1e-6 relative
-residual and 1e-6 relative compliance against FP64.
+Tolerances are written in each test.  The bar is 1e-6 relative residual and
+1e-6 relative compliance against FP64.
 """
 import time
 
@@ -96,7 +96,7 @@ def _check_against(res, ref):
         c_err = abs(c - c_ref) / abs(c_ref)
         u_err = np.linalg.norm(ug - u_ref) / np.linalg.norm(u_ref)
         _say(f"{name}: compliance error {c_err:.3e}, displacement error {u_err:.3e}")
-        # compliance to the north star's 1e-6; the displacement error is set
+        # compliance to 1e-6 relative; the displacement error is set
         # by the 1e-6 L-inf stopping test (about 3e-4 here in both precisions)
         assert c_err <= 1e-6, name
         assert u_err <= 2e-3, name
