@@ -543,6 +543,34 @@ def test_fp64_solve_matches_oracle(levels):
     assert ig["r_linf"] <= tol * ig["r0_linf"]
 
 
+@pytest.mark.parametrize("smooth,bnd", [(2, 5), (1, 0), (3, 1)])
+def test_smoothing_counts_follow_the_oracle(smooth, bnd):
+    """pre_and_post_smoothing_iter and boundary_smoothing other than the
+    caller's 1 and 3 (the reference interface's default boundary count is
+    5, fem_interface.h:298-300): V-cycles in both precisions and the FP64
+    solve match the oracle with the same counts"""
+    import dataclasses
+    prob = dataclasses.replace(W.random_blocky(seed=23, n=(16, 12, 12), mg_levels=4), smooth_iters=smooth,
+                               boundary_iters=bnd)
+    P = Pair(prob)
+    rng = np.random.default_rng(17)
+    n = len(P.map[0][0])
+    gf = P.map[0][1]
+    b = rng.standard_normal((n, 3))
+    b[(gf & 1) == 0] = 0
+    b[((gf[:, None] >> (1 + np.arange(3))) & 1).astype(bool)] = 0
+    want = P.from_o(0, P.o.vcycle(P.to_o(0, b), O.MODE_FP64))
+    for prec, tol in ((1, 1e-10), (0, 1e-4)):
+        got = P.g.kernel(_lib.OP_VCYCLE, 0, None, b, precision=prec)
+        assert rel(got, want) < tol, (prec, rel(got, want))
+    u_o, _, io = P.o.solve(tolerance=1e-10, max_iterations=500)
+    out, ig = P.g.solve(tolerance=1e-10, max_iterations=500, precision=1)
+    print("iterations: oracle", io["iterations"], "gpu", ig["iterations"], flush=True)
+    assert ig["converged"] == 1
+    assert abs(ig["iterations"] - io["iterations"]) <= max(2, io["iterations"] // 20)
+    assert _solution_error(P, out, u_o) < 1e-7
+
+
 def test_bottom_with_more_than_2_31_inverse_entries(monkeypatch):
     """a bottom level of more than 46341 unknowns (its dense inverse has more
     than 2^31 entries and takes the 64-bit factorization path) still solves
