@@ -116,6 +116,7 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   sync();
   const double t_upload = now_s();
   build_operators();
+  setup_colour_major();
   sync();
   size_t free1 = 0;
   HIPCHK(hipMemGetInfo(&free1, &total0));
@@ -245,14 +246,6 @@ void Solver::upload_layout() {
     R.u64.alloc(R.n3 * 8);
     R.b64.alloc(R.n3 * 8);
     R.r64.alloc(R.n3 * 8);
-  }
-  // level-0 flags in colour-major order, and room for b in that order
-  if (cmb_mode_ == 1 || (cmb_mode_ < 0 && colour_major_by_size(lv_[0].d.nb))) {
-    const LevelRT& R0 = lv_[0];
-    fcm_.alloc((size_t)R0.d.nb * 64);
-    launch_colour_major_flags(st_, R0.d.nb, R0.d.flags, fcm_.as<uint8_t>());
-    lv_[0].d.flags_cm = fcm_.as<uint8_t>();
-    bcm_.alloc((size_t)R0.n3 * 8);
   }
   // level-0 cell data
   upload(E64_, lay_.E);
@@ -587,6 +580,25 @@ double Solver::read_scalar(const double* dev) {
 // the copy; the 512x256x256 band (245,910 blocks) 4.254 / 5.937 ms against
 // 4.100 / 5.809 ms.  By default the copy is used above 256 MiB of u, b, E and
 // neighbour table in FP32, i.e. above about 137 K blocks.
+// The level-0 flags in colour-major order and room for b in that order,
+// allocated after everything the solve needs: without the memory for them
+// the colour sweeps keep reading node-major.
+void Solver::setup_colour_major() {
+  if (!(cmb_mode_ == 1 || (cmb_mode_ < 0 && colour_major_by_size(lv_[0].d.nb)))) return;
+  const LevelRT& R0 = lv_[0];
+  try {
+    fcm_.alloc((size_t)R0.d.nb * 64);
+    bcm_.alloc((size_t)R0.n3 * 8);
+  } catch (const SpgError&) {
+    (void)hipGetLastError();
+    fcm_.alloc(0);
+    bcm_.alloc(0);
+    return;
+  }
+  launch_colour_major_flags(st_, R0.d.nb, R0.d.flags, fcm_.as<uint8_t>());
+  lv_[0].d.flags_cm = fcm_.as<uint8_t>();
+}
+
 template <class T>
 const T* Solver::colour_major_b(const T* b0) {
   if (!lv_[0].d.flags_cm) return nullptr;

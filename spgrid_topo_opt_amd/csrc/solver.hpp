@@ -106,6 +106,7 @@ class Solver {
   // when the level's working set exceeds the memory-side cache (or
   // SPG_GS_CMB=1); returns the copy of b0 to pass, or null
   template <class T> const T* colour_major_b(const T* b0);
+  void setup_colour_major();
   int cmb_mode_ = -1;  // SPG_GS_CMB: 0 never, 1 always, unset by size
   template <class T> void level_residual(int l, const T* u, const T* b, T* r);
   template <class T> void bottom(const T* b, T* u);
