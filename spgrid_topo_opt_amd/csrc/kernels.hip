@@ -2042,6 +2042,7 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const i
   const int blk = xcd_chunk(blockIdx.x, gridDim.x) * TR_WAVES + w;
   const bool live = blk < nbc;
   if (lane < 27) ht[w][lane] = live ? rsrc[blk * 27 + lane] : -1;
+  const uint8_t fl = live ? cflags[blk * 64 + lane] : 0;  // with the source table, not after the sums
   __syncthreads();
   {
     // the 9^3 fine halo in 12 passes; all loads are issued before the LDS writes
@@ -2079,7 +2080,6 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const i
         a1 += wgt * hf[w][1][h];
         a2 += wgt * hf[w][2][h];
       }
-  const uint8_t fl = cflags[blk * 64 + lane];
   coarse[vidx(blk, 0, lane)] = (fl & 2) ? T(0) : a0;
   coarse[vidx(blk, 1, lane)] = (fl & 4) ? T(0) : a1;
   coarse[vidx(blk, 2, lane)] = (fl & 8) ? T(0) : a2;
@@ -2088,9 +2088,30 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_restrict_lds(int nbc, const i
     for (int c = 0; c < 3; ++c) czero[vidx(blk, c, lane)] = T(0);
 }
 
+// The 27 coarse nodes around fine block b: the 3^3 coarse nodes from the
+// fine block's phase (((base + B) >> 1) & 3 per axis) in the up to 8 coarse
+// blocks of its psrc row, as (coarse block << 6) | local index, -1 absent.
+// Built once per topology, so the prolongation's coarse loads depend on one
+// table load instead of the base, then psrc, then the data.
+__device__ __forceinline__ int prolong_source(int b, int lane, const int* __restrict__ psrc,
+                                              const int* __restrict__ fbase) {
+  const int phx = ((fbase[3 * b] + 4) >> 1) & 3, phy = ((fbase[3 * b + 1] + 4) >> 1) & 3,
+            phz = ((fbase[3 * b + 2] + 8) >> 1) & 3;
+  const int x = phx + lane / 9, y = phy + (lane / 3) % 3, z = phz + lane % 3;
+  const int cb = psrc[b * 8 + ((x >> 2) << 2) + ((y >> 2) << 1) + (z >> 2)];
+  return cb >= 0 ? (cb << 6) | loc3(x, y, z) : -1;
+}
+__global__ void k_prolong_table(int nbf, const int* __restrict__ psrc, const int* __restrict__ fbase,
+                                int* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)nbf * 27) return;
+  out[i] = prolong_source((int)(i / 27), (int)(i % 27), psrc, fbase);
+}
+
 template <class T>
 __global__ void __launch_bounds__(64 * TR_WAVES) k_prolong_lds(int nbf, const int* __restrict__ psrc,
                                                                const int* __restrict__ fbase,
+                                                               const int* __restrict__ psrc27,
                                                                const uint8_t* __restrict__ fflags,
                                                                const T* __restrict__ coarse, T* __restrict__ fine) {
   __shared__ T hc[TR_WAVES][3][27];
@@ -2102,14 +2123,10 @@ __global__ void __launch_bounds__(64 * TR_WAVES) k_prolong_lds(int nbf, const in
   // their loads go out first
   const uint8_t fl = fflags[b0 * 64 + lane];
   T a0 = fine[vidx(b0, 0, lane)], a1 = fine[vidx(b0, 1, lane)], a2 = fine[vidx(b0, 2, lane)];
-  const int phx = ((fbase[3 * b0] + 4) >> 1) & 3, phy = ((fbase[3 * b0 + 1] + 4) >> 1) & 3,
-            phz = ((fbase[3 * b0 + 2] + 8) >> 1) & 3;
   if (lane < 27) {
-    const int x = phx + lane / 9, y = phy + (lane / 3) % 3, z = phz + lane % 3;
-    const int cb = live ? psrc[blk * 8 + ((x >> 2) << 2) + ((y >> 2) << 1) + (z >> 2)] : -1;
-    const int q = loc3(x, y, z);
+    const int t = !live ? -1 : (psrc27 ? psrc27[blk * 27 + lane] : prolong_source(blk, lane, psrc, fbase));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) hc[w][c][lane] = cb >= 0 ? coarse[vidx(cb, c, q)] : T(0);
+    for (int c = 0; c < 3; ++c) hc[w][c][lane] = t >= 0 ? coarse[vidx(t >> 6, c, t & 63)] : T(0);
   }
   __syncthreads();
   if (!live) return;
@@ -3075,8 +3092,11 @@ void launch_restrict(hipStream_t st, const LevelDev& C, const T* fine, T* coarse
 }
 template <class T>
 void launch_prolongate(hipStream_t st, const LevelDev& F, const T* coarse, T* fine) {
-  k_prolong_lds<T><<<(F.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(F.nb, F.psrc, F.base, F.flags, coarse,
-                                                                              fine);
+  k_prolong_lds<T><<<(F.nb + TR_WAVES - 1) / TR_WAVES, 64 * TR_WAVES, 0, st>>>(F.nb, F.psrc, F.base, F.psrc27, F.flags,
+                                                                              coarse, fine);
+}
+void launch_prolong_table(hipStream_t st, const LevelDev& F, int* out) {
+  if (F.nb > 0) k_prolong_table<<<nblk((int64_t)F.nb * 27), TPB, 0, st>>>(F.nb, F.psrc, F.base, out);
 }
 template <class T>
 void launch_zero(hipStream_t st, T* v, int64_t n) {

@@ -18,6 +18,9 @@ struct LevelDev {
   const uint8_t* flags = nullptr;  // [nb*64]
   const int* rsrc = nullptr;       // [nb*27] restriction sources (level >= 1)
   const int* psrc = nullptr;       // [nb*8]  prolongation sources (level < L-1)
+  // [nb*27] the 27 coarse nodes a fine block interpolates from, as
+  // (coarse block << 6) | local index, -1 where absent (launch_prolong_table)
+  const int* psrc27 = nullptr;
   const int* base = nullptr;       // [nb*3]  block origin (node coords)
   const int* bnd = nullptr;        // boundary nodes (blk*64+l), concatenated by colour
   int bnd_off[9] = {};             // colour c owns bnd[bnd_off[c] .. bnd_off[c+1])
@@ -107,6 +110,8 @@ void launch_gs0(hipStream_t st, const LevelDev& L, const T* E, T dx, T* u, const
 // k-th node of colour c in block blk (level-0 vectors)
 template <class T>
 void launch_colour_major(hipStream_t st, const LevelDev& L, const T* v, T* vcm);
+// fills the level's psrc27 table (out, [nb*27]) from its psrc and base
+void launch_prolong_table(hipStream_t st, const LevelDev& F, int* out);
 void launch_colour_major_flags(hipStream_t st, int nb, const uint8_t* flags, uint8_t* fcm);
 // one level-0 colour step over a list of slots (27 neighbour slots each in nslot)
 template <class T>

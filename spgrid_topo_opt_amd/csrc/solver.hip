@@ -240,6 +240,11 @@ void Solver::upload_layout() {
     R.d.base = R.base.as<int>();
     R.d.rsrc = R.rsrc.as<int>();
     R.d.psrc = R.psrc.as<int>();
+    if (R.psrc.p) {
+      R.psrc27.alloc((size_t)D.nb * 27 * 4);
+      launch_prolong_table(st_, R.d, R.psrc27.as<int>());
+      R.d.psrc27 = R.psrc27.as<int>();
+    }
     R.u32.alloc(R.n3 * 4);
     R.b32.alloc(R.n3 * 4);
     R.r32.alloc(R.n3 * 4);

@@ -47,7 +47,7 @@ struct DBuf {
 
 struct LevelRT {
   LevelDev d;
-  DBuf nbr, flags, rsrc, psrc, base, bnd;
+  DBuf nbr, flags, rsrc, psrc, psrc27, base, bnd;
   DBuf st64, st32, cell_row, cfb, cgi;
   DBuf bci, bslot, bA64, bA32, bt64, bt32;  // boundary system
   DBuf b0cslot, b0node, b0E64, b0E32, b0D64, b0D32;  // level-0 matrix-free boundary system
