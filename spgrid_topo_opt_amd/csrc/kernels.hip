@@ -51,14 +51,13 @@ static int host_entry(int x, int y, int z) {
   return (sp << 8) | q;
 }
 
-void upload_k0(const double* K0, double mu, double lambda, hipStream_t st) {
+hipError_t upload_k0(const double* K0, double mu, double lambda) {
   g_cell = CellConsts{9.0 * mu / 72.0,  4.5 * lambda / 72.0, 4.5 * mu / 72.0,
                       1.5 * mu / 72.0,  1.5 * lambda / 72.0, (2.0 * mu + 0.5 * lambda) / 72.0};
   float f[576];
   for (int i = 0; i < 576; ++i) f[i] = (float)K0[i];
-  (void)st;
-  hipMemcpyToSymbol(HIP_SYMBOL(cK0d), K0, sizeof(double) * 576, 0, hipMemcpyHostToDevice);
-  hipMemcpyToSymbol(HIP_SYMBOL(cK0f), f, sizeof(float) * 576, 0, hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cK0d), K0, sizeof(double) * 576, 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cK0f), f, sizeof(float) * 576, 0, hipMemcpyHostToDevice);
   int h6[216], h5[125], hc[8 * 125];
   for (int h = 0; h < 216; ++h) h6[h] = host_entry(h / 36 - 1, (h / 6) % 6 - 1, h % 6 - 1);
   for (int h = 0; h < 125; ++h) h5[h] = host_entry(h / 25 - 1, (h / 5) % 5 - 1, h % 5 - 1);
@@ -66,9 +65,10 @@ void upload_k0(const double* K0, double mu, double lambda, hipStream_t st) {
     const int px = (c >> 2) & 1, py = (c >> 1) & 1, pz = c & 1;
     for (int h = 0; h < 125; ++h) hc[c * 125 + h] = host_entry(px - 1 + h / 25, py - 1 + (h / 5) % 5, pz - 1 + h % 5);
   }
-  hipMemcpyToSymbol(HIP_SYMBOL(cHalo6), h6, sizeof(h6), 0, hipMemcpyHostToDevice);
-  hipMemcpyToSymbol(HIP_SYMBOL(cHalo5), h5, sizeof(h5), 0, hipMemcpyHostToDevice);
-  hipMemcpyToSymbol(HIP_SYMBOL(cHaloC), hc, sizeof(hc), 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cHalo6), h6, sizeof(h6), 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cHalo5), h5, sizeof(h5), 0, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cHaloC), hc, sizeof(hc), 0, hipMemcpyHostToDevice);
+  return e;
 }
 
 // fused multiply-add in the working precision; row sums are written as FMA

@@ -89,7 +89,10 @@ struct CgScalars {
   int pad;
 };
 
-void upload_k0(const double* K0, double mu, double lambda, hipStream_t st);
+// The element matrix K0 = mu K_mu + lambda K_la and the halo tables, in the
+// device's constant memory (one set per device: Solver::activate uploads
+// its material's before its kernels run).  Synchronous.
+hipError_t upload_k0(const double* K0, double mu, double lambda);
 
 // level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
 template <class T>

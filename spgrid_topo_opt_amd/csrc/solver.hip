@@ -110,7 +110,7 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
   double kmu[576], kla[576];
   element_tables(kmu, kla);
   for (int e = 0; e < 576; ++e) K0_[e] = cfg.mu * kmu[e] + cfg.lambda * kla[e];
-  upload_k0(K0_, cfg.mu, cfg.lambda, st_);
+  activate();
   const double t_init = now_s();
   upload_layout();
   sync();
@@ -137,6 +137,35 @@ Solver::~Solver() {
 }
 
 void Solver::sync() { HIPCHK(hipStreamSynchronize(st_)); }
+
+// The element constants live in device constant memory, one set per device,
+// shared by every solver on it.  The last material uploaded is recorded per
+// device; a solver whose material differs waits for the device (work of the
+// other solver may still read the constants) and uploads its own.  Calls
+// into solvers of different materials on one device from several threads
+// at once are not supported.
+void Solver::activate() {
+  struct Key {
+    bool set = false;
+    double mu = 0, lambda = 0;
+  };
+  static std::mutex mu;
+  static std::vector<Key> keys;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)keys.size() <= cfg_.device) keys.resize(cfg_.device + 1);
+  Key& k = keys[cfg_.device];
+  if (k.set && k.mu == cfg_.mu && k.lambda == cfg_.lambda) return;
+  int prev = 0;
+  HIPCHK(hipGetDevice(&prev));
+  HIPCHK(hipSetDevice(cfg_.device));
+  HIPCHK(hipDeviceSynchronize());
+  const hipError_t e = upload_k0(K0_, cfg_.mu, cfg_.lambda);
+  HIPCHK(hipSetDevice(prev));
+  if (e != hipSuccess) throw SpgError(SPG_E_HIP, std::string("element constants: ") + hipGetErrorString(e));
+  k.set = true;
+  k.mu = cfg_.mu;
+  k.lambda = cfg_.lambda;
+}
 
 void Solver::check_coop() {
   if (lv_.empty() || !lv_[0].coop.p) return;
@@ -498,6 +527,7 @@ bool Solver::active_set_matches(const double* density) const {
 }
 
 void Solver::update_density(const double* density) {
+  activate();
   // recompute E (SIMP) for the existing cells, then the coarse operators
   const Layout& Lc = lay_;
   std::vector<double> E(Lc.E.size(), 0.0), rho(Lc.rho.size(), 0.0);
@@ -525,6 +555,7 @@ void Solver::update_density(const double* density) {
 }
 
 void Solver::update_density_device(const double* density_dev) {
+  activate();
   // same checks and SIMP map as update_density, on the device; nothing
   // changes when the check fails
   const int64_t nin = lay_.nin_blocks * 64;
@@ -961,6 +992,7 @@ void Solver::static_info(spg_info* info) const {
 }
 
 int Solver::solve(const spg_solve_params& prm, spg_info* info) {
+  activate();
   const LevelRT& R0 = lv_[0];
   const int64_t n = R0.n3;
   if (prm.warm_start < 0 || prm.warm_start > 2) throw SpgError(SPG_E_INVALID, "warm_start must be 0, 1 or 2");
@@ -991,6 +1023,7 @@ void Solver::set_initial_guess_in3(const double* dev) {
 }
 
 int Solver::solve_adjoint(const spg_solve_params& prm, const double* g_in3, double* lam_in3, spg_info* info) {
+  activate();
   const LevelRT& R0 = lv_[0];
   const int64_t n = R0.n3;
   const int64_t nin = lay_.nin_blocks * 64;
@@ -1041,6 +1074,7 @@ void Solver::displacement_in3(double* dev) const {
 }
 
 double Solver::sensitivity(const double* u_in3_dev, const double* lam_in3_dev, double* dc_in_dev) {
+  activate();
   const LevelRT& R0 = lv_[0];
   const int64_t nslot = (int64_t)R0.d.nb * 64;
   const int64_t nin = lay_.nin_blocks * 64;
@@ -1093,6 +1127,7 @@ static void from_dev_layout(const std::vector<T>& v, int64_t nb, double* out) {
 
 void Solver::run_kernel(int op, int level, int precision, int forward, const double* in, const double* b,
                         double* out) {
+  activate();
   if (level < 0 || level >= lay_.nlev) throw SpgError(SPG_E_INVALID, "level out of range");
   auto body = [&](auto tag) {
     using T = decltype(tag);
@@ -1209,6 +1244,7 @@ void Solver::level_stencils(int level, double* out) {
 }
 
 double Solver::bench_apply(int precision, int reps) {
+  activate();
   const LevelRT& R0 = lv_[0];
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
@@ -1237,6 +1273,7 @@ double Solver::bench_apply(int precision, int reps) {
 }
 
 double Solver::bench_kernel(int op, int level, int precision, int arg, int reps) {
+  activate();
   if (level < 0 || level >= lay_.nlev) throw SpgError(SPG_E_INVALID, "level out of range");
   const int B = lay_.nlev - 1;
   hipEvent_t e0, e1;

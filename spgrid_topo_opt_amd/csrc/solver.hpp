@@ -121,6 +121,9 @@ class Solver {
   int solve_rhs(const spg_solve_params& prm, double* u, const double* f, spg_info* info, bool warm);
   void reset_to_ud(double* u);
   void build_operators();
+  // makes the device's element constants this solver's material (uploaded
+  // only when another material was the last one used on the device)
+  void activate();
   void build_boundary_rows();
   void upload_layout();
   double read_scalar(const double* dev);
