@@ -18,12 +18,10 @@ namespace spg {
 __constant__ double cK0d[576];
 __constant__ float cK0f[576];
 
-// constants of the strain-form cell operator per unit modulus: 9mu, 4.5la,
-// 4.5mu, 1.5mu, 1.5la and 2mu+0.5la, all / 72 (k_apply_cell)
-struct CellConsts {
-  double d, t, s, w, q, c3;
-};
-static CellConsts g_cell;
+CellConsts cell_consts(double mu, double lambda) {
+  return CellConsts{9.0 * mu / 72.0,  4.5 * lambda / 72.0, 4.5 * mu / 72.0,
+                    1.5 * mu / 72.0,  1.5 * lambda / 72.0, (2.0 * mu + 0.5 * lambda) / 72.0};
+}
 
 template <class T>
 struct KT;
@@ -51,9 +49,7 @@ static int host_entry(int x, int y, int z) {
   return (sp << 8) | q;
 }
 
-hipError_t upload_k0(const double* K0, double mu, double lambda) {
-  g_cell = CellConsts{9.0 * mu / 72.0,  4.5 * lambda / 72.0, 4.5 * mu / 72.0,
-                      1.5 * mu / 72.0,  1.5 * lambda / 72.0, (2.0 * mu + 0.5 * lambda) / 72.0};
+hipError_t upload_k0(const double* K0) {
   float f[576];
   for (int i = 0; i < 576; ++i) f[i] = (float)K0[i];
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cK0d), K0, sizeof(double) * 576, 0, hipMemcpyHostToDevice);
@@ -359,7 +355,7 @@ __global__ void __launch_bounds__(64 * AP_WAVES) k_apply_lds(int nb, const int* 
 // before adding the pairs, so rigid translations and rotations cancel
 // exactly and FP32 sums keep the accuracy of the row sums
 // (SPGrid_Linear_Elasticity.h:22-213 computes the same product node-wise).
-// (constants of the strain-form cell operator: CellConsts, set by upload_k0)
+// (constants of the strain-form cell operator: CellConsts, per material in LevelDev::cell)
 template <class T>
 struct CellK {
   T d, t, s, w, q, c3;
@@ -2701,7 +2697,7 @@ template <class T>
 static void launch_apply_cell(hipStream_t st, const LevelDev& L, const T* E, T dx, const T* in, const T* b, T* out,
                               int mode, double* dpart = nullptr) {
   const int nwg = (L.nb + AP_WAVES - 1) / AP_WAVES;
-  const CellConsts k = g_cell;
+  const CellConsts k = L.cell;
   if (mode == 0)
     k_apply_cell<T, 0><<<nwg, 64 * AP_WAVES, 0, st>>>(L.nb, L.nbr, L.flags, E, k, dx, in, b, out);
   else if (mode == 1)
@@ -2798,12 +2794,9 @@ static bool launch_bnd_phase(hipStream_t st, const LevelDev& L, T* u, const T* b
   T* bt = BndPtr<T>::bt(L);
   if (!A || !bt || !L.bci || !L.bslot || lds > kMaxLds) return false;
   constexpr int NT = BndThreads<T>::value;
-  static const bool attr = [] {
-    (void)hipFuncSetAttribute((const void*)k_bnd_sweeps<T, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kMaxLds);
-    return true;
-  }();
-  (void)attr;
+  // the attribute is per device: set at every launch (a host-side call)
+  (void)hipFuncSetAttribute((const void*)k_bnd_sweeps<T, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kMaxLds);
   k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt);
   BndOff o;
   for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];
@@ -2834,11 +2827,7 @@ static bool launch_bnd0_phase(hipStream_t st, const LevelDev& L, T dx, T* u, con
   const T* bE = Bnd0Ptr<T>::E(L);
   const T* bD = Bnd0Ptr<T>::D(L);
   if (!A || !bt || !bE || !bD || !L.b0node || !L.bslot || lds > kMaxLds) return false;
-  static const bool attr = [] {
-    (void)hipFuncSetAttribute((const void*)k_bnd0_sweeps<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-    return true;
-  }();
-  (void)attr;
+  (void)hipFuncSetAttribute((const void*)k_bnd0_sweeps<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   k_bnd_rhs<T><<<(n + 31) / 32, 256, 0, st>>>(n, L.bnd, L.bslot, A, u, b, bt);
   BndOff o;
   for (int c = 0; c < 9; ++c) o.off[c] = L.bnd_off[c];

@@ -12,8 +12,16 @@
 
 namespace spg {
 
+// constants of the strain-form cell operator per unit modulus: 9mu, 4.5la,
+// 4.5mu, 1.5mu, 1.5la and 2mu+0.5la, all / 72 (k_apply_cell)
+struct CellConsts {
+  double d = 0, t = 0, s = 0, w = 0, q = 0, c3 = 0;
+};
+CellConsts cell_consts(double mu, double lambda);
+
 struct LevelDev {
   int nb = 0;
+  CellConsts cell;                 // level 0: the material's cell-operator constants
   const int* nbr = nullptr;        // [nb*27]
   const uint8_t* flags = nullptr;  // [nb*64]
   const int* rsrc = nullptr;       // [nb*27] restriction sources (level >= 1)
@@ -92,7 +100,7 @@ struct CgScalars {
 // The element matrix K0 = mu K_mu + lambda K_la and the halo tables, in the
 // device's constant memory (one set per device: Solver::activate uploads
 // its material's before its kernels run).  Synchronous.
-hipError_t upload_k0(const double* K0, double mu, double lambda);
+hipError_t upload_k0(const double* K0);
 
 // level-0 operator: out = K in (mode 0), out = P(b - K in) (mode 1), out = P K in (mode 2)
 template <class T>

@@ -159,7 +159,7 @@ void Solver::activate() {
   HIPCHK(hipGetDevice(&prev));
   HIPCHK(hipSetDevice(cfg_.device));
   HIPCHK(hipDeviceSynchronize());
-  const hipError_t e = upload_k0(K0_, cfg_.mu, cfg_.lambda);
+  const hipError_t e = upload_k0(K0_);
   HIPCHK(hipSetDevice(prev));
   if (e != hipSuccess) throw SpgError(SPG_E_HIP, std::string("element constants: ") + hipGetErrorString(e));
   k.set = true;
@@ -190,6 +190,7 @@ void Solver::upload_layout() {
     const LevelDesc& D = lay_.lv[l];
     LevelRT& R = lv_[l];
     R.d.nb = (int)D.nb;
+    R.d.cell = cell_consts(cfg_.mu, cfg_.lambda);
     R.n3 = D.nb * 192;
     upload(R.nbr, D.nbr);
     upload(R.flags, D.flags);
