@@ -304,7 +304,8 @@ static bool same_topology(const spg_solver* h, const FemInputs& in, const std::v
          c.dx == in.dx && c.mu == in.global_mu && c.lambda == in.global_lambda &&
          c.penalty == in.get_double("penalty", 0.0) && c.min_stiffness == in.get_double("minimum_stiffness", 0.0) &&
          c.min_density == in.get_double("minimum_density", 0.0) && c.mg_levels == in.mg_level &&
-         c.smooth_iters == in.pre_and_post_smoothing_iter && c.boundary_iters == in.get_int("boundary_smoothing", 5);
+         c.smooth_iters == in.pre_and_post_smoothing_iter && c.boundary_iters == in.get_int("boundary_smoothing", 5) &&
+         c.device == in.get_int("device", 0);
 }
 
 // The previous displacement mapped onto a new block list by block
