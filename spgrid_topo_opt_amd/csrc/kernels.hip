@@ -2640,13 +2640,12 @@ __global__ void k_gather_input_scalar(const int* in_slot, int64_t nin, const dou
   const int s = in_slot[i];
   out[i] = s >= 0 ? v[s] : 0.0;
 }
-__global__ void k_set_entries(double* v, const int* idx, const double* val, int64_t n) {
-  // serial order (last write wins) across duplicates: one thread
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int64_t i = 0; i < n; ++i) {
-    const int s = idx[i] / 3, c = idx[i] % 3;
-    v[vidx(s >> 6, c, s & 63)] = val[i];
-  }
+// the entries are distinct (the host keeps one per slot and component)
+__global__ void k_set_entries(double* v, const int* __restrict__ idx, const double* __restrict__ val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = idx[i] / 3, c = idx[i] % 3;
+  v[vidx(s >> 6, c, s & 63)] = val[i];
 }
 
 // ======================================================== launch wrappers
@@ -3221,7 +3220,7 @@ void launch_gather_input_scalar(hipStream_t st, const int* in_slot, int64_t nin,
   if (nin > 0) k_gather_input_scalar<<<nblk(nin), TPB, 0, st>>>(in_slot, nin, v, out);
 }
 void launch_set_entries(hipStream_t st, double* v, const int* idx, const double* val, int64_t n) {
-  if (n > 0) k_set_entries<<<1, 64, 0, st>>>(v, idx, val, n);
+  if (n > 0) k_set_entries<<<nblk(n), TPB, 0, st>>>(v, idx, val, n);
 }
 
 #define SPG_INST(T)                                                                                              \

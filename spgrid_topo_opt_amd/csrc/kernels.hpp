@@ -225,6 +225,7 @@ void launch_sensitivity(hipStream_t st, const LevelDev& L, const double* rho, co
 void launch_scatter_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* in3, double* v);
 void launch_gather_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out3);
 void launch_gather_input_scalar(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out);
+// v[idx[i]] = val[i] (entry = slot * 3 + component) in parallel: the entries must be distinct
 void launch_set_entries(hipStream_t st, double* v, const int* idx, const double* val, int64_t n);
 
 }  // namespace spg

@@ -815,11 +815,20 @@ Layout build_layout(const ProblemDesc& p) {
         const int32_t* bb = p.block_base + 3 * b;
         in_nodes_off[bkey(bb[0] + (l >> 4), bb[1] + ((l >> 2) & 3), bb[2] + (l & 3))].push_back(i);
       }
+    // one entry per (slot, axis): a repeated one keeps its first place and
+    // the last value, as writing the list in order would leave it
+    std::unordered_map<int32_t, size_t> ud_at;
     for (int64_t i = 0; i < p.ndir; ++i) {
       int x = p.dir_coord[3 * i], y = p.dir_coord[3 * i + 1], z = p.dir_coord[3 * i + 2];
       const int32_t gb = gmap[0].find(x >> 2, y >> 2, z >> 2);
       if (gb >= 0) {
-        out.ud_slot.push_back((int32_t)((gb * 64 + loc(x, y, z)) * 3 + p.dir_axis[i]));
+        const int32_t e = (int32_t)((gb * 64 + loc(x, y, z)) * 3 + p.dir_axis[i]);
+        auto [pos, fresh] = ud_at.emplace(e, out.ud_slot.size());
+        if (!fresh) {
+          out.ud_value[pos->second] = p.dir_value[i];
+          continue;
+        }
+        out.ud_slot.push_back(e);
         out.ud_value.push_back(p.dir_value[i]);
       } else {
         auto jt = in_nodes_off.find(bkey(x, y, z));

@@ -206,7 +206,7 @@ struct Layout {
   int64_t nin_blocks = 0;
   NodeArray<int32_t> in_node_slot;   // [nin_blocks*64]
   // Dirichlet values u_D at level-0 slots, and at input nodes outside the GPU set
-  std::vector<int32_t> ud_slot;        // slot*3+axis
+  std::vector<int32_t> ud_slot;        // slot*3+axis, each at most once
   std::vector<double> ud_value;
   // input nodes outside the GPU blocks that carry a Dirichlet value:
   // (input node index)*3+axis and value, in input order (last write wins)

@@ -585,14 +585,21 @@ void Solver::set_forces(int64_t n, const int32_t* coord, const double* value) {
   }
   std::vector<int32_t> idx;
   std::vector<double> val;
+  std::unordered_map<int32_t, size_t> at;  // node slot -> its entry: a repeated node keeps the last value
   for (int64_t i = 0; i < n; ++i) {
     const int x = coord[3 * i], y = coord[3 * i + 1], z = coord[3 * i + 2];
     auto it = gmap0_.find(bkey(x >> 2, y >> 2, z >> 2));
     if (it == gmap0_.end()) continue;  // forces outside the solved blocks are ignored
     const int32_t g = it->second;
     const int q = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+    const int32_t slot = g * 64 + q;
+    auto [pos, fresh] = at.emplace(slot, idx.size());
+    if (!fresh) {
+      for (int c = 0; c < 3; ++c) val[pos->second + c] = value[3 * i + c];
+      continue;
+    }
     for (int c = 0; c < 3; ++c) {
-      idx.push_back((g * 64 + q) * 3 + c);
+      idx.push_back(slot * 3 + c);
       val.push_back(value[3 * i + c]);
     }
   }

@@ -571,6 +571,34 @@ def test_smoothing_counts_follow_the_oracle(smooth, bnd):
     assert _solution_error(P, out, u_o) < 1e-7
 
 
+def test_repeated_force_and_dirichlet_entries_keep_the_last():
+    """a force node or a Dirichlet (node, axis) listed twice takes its last
+    value, as the reference's in-order writes leave it
+    (SPGrid_CPU_Solver.cpp:241-246, 289-296); the solve matches the oracle
+    and the solve of the lists with the repeats removed, bit for bit"""
+    import dataclasses
+    p = W.box(16, 8, 8, seed=2, mg_levels=3)
+    fc, fv = p.force_coord, p.force_value
+    dc, da, dv = p.dir_coord, p.dir_axis, p.dir_value
+    rep = dataclasses.replace(
+        p, force_coord=np.concatenate([fc, fc[:2]]),
+        force_value=np.concatenate([fv, fv[:2] * np.array([[2.0, -1.0, 0.5]])]),
+        dir_coord=np.concatenate([dc, dc[:3]]), dir_axis=np.concatenate([da, da[:3]]),
+        dir_value=np.concatenate([dv, np.array([1e-4, -2e-4, 3e-4])]))
+    once = dataclasses.replace(
+        p, force_value=np.concatenate([fv[:2] * np.array([[2.0, -1.0, 0.5]]), fv[2:]]),
+        dir_value=np.concatenate([np.array([1e-4, -2e-4, 3e-4]), dv[3:]]))
+    kw = dict(tolerance=1e-10, precision=1, max_iterations=500)
+    P = Pair(rep)
+    out, info = P.g.solve(**kw)
+    out_once, info_once = _lib.Solver(once).solve(**kw)
+    assert info["iterations"] == info_once["iterations"]
+    assert np.array_equal(out, out_once)
+    u_o, _, io = P.o.solve(tolerance=1e-12, max_iterations=500)
+    assert io["rc"] == 0
+    assert _solution_error(P, out, u_o) < 1e-8
+
+
 def test_bottom_with_more_than_2_31_inverse_entries(monkeypatch):
     """a bottom level of more than 46341 unknowns (its dense inverse has more
     than 2^31 entries and takes the 64-bit factorization path) still solves
