@@ -1,7 +1,10 @@
 // capi.hip -- extern "C" entry points (include/spgrid_topo_opt_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -21,7 +24,7 @@ using namespace spg;
 // the first solve (state.initial_residual_linf, :318-326).
 struct WarmState {
   std::vector<int32_t> base;  // [nb*3] input blocks of the last solve
-  std::vector<double> u;      // [nb*64*3] their displacement
+  NodeArray<double> u;        // [nb*64*3] their displacement
   double initial_linf = 0.0;
   bool valid = false;
 };
@@ -37,6 +40,9 @@ struct spg_solver {
 };
 
 static thread_local std::string g_err;
+
+// a displacement block of the FEMOutputs stream: int32 base[3], 64 x 4 doubles
+constexpr size_t kRecordBytes = 12 + 64 * 32;
 
 #define API_TRY try {
 #define API_CATCH                          \
@@ -324,9 +330,29 @@ static std::vector<double> carry_over(const WarmState& w, const std::vector<int3
   return u;
 }
 
+namespace {
+// SPG_SETUP_TRACE=2: seconds per stage of fem_solve on stderr
+struct StageTrace {
+  bool on = false;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  StageTrace() {
+    const char* e = std::getenv("SPG_SETUP_TRACE");
+    on = e && std::atoi(e) >= 2;
+  }
+  void operator()(const char* what) {
+    if (!on) return;
+    const auto n = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[spg] fem_solve %-24s %.3f s\n", what, std::chrono::duration<double>(n - t).count());
+    t = n;
+  }
+};
+}  // namespace
+
 // Drop-in for `fem_solve` (topo_opt/fem_interface.h:11; solver/SPGrid_CPU_Solver.cpp:26-409).
 void fem_solve(void* input, void* output) {
+  StageTrace trace;
   FemOutputs out;
+  std::unique_ptr<char[]> emitted;  // the finished stream of a solve that ran
   out.success = false;
   if (!output) return;
   // the kept state: returned to the caller when keep_state is set (also
@@ -338,6 +364,7 @@ void fem_solve(void* input, void* output) {
     FemInputs in;
     WireReader r(input);
     read_inputs(r, in);
+    trace("read inputs");
     h.reset(static_cast<spg_solver*>(in.get_state_ptr()));
     keep = in.get_int("keep_state", 0) != 0;
     for (int d = 0; d < 3; ++d) out.displacements.resolution[d] = in.resolution[d];
@@ -349,11 +376,16 @@ void fem_solve(void* input, void* output) {
     }
     const size_t nb = in.density.blocks.size();
     std::vector<int32_t> base(nb * 3);
-    std::vector<double> dens(nb * 64);
-    for (size_t b = 0; b < nb; ++b) {
-      for (int c = 0; c < 3; ++c) base[3 * b + c] = in.density.blocks[b].base[c];
-      std::memcpy(&dens[64 * b], in.density.blocks[b].data, 64 * sizeof(double));
-    }
+    NodeArray<double> dens(nb * 64);
+    parallel_ranges(
+        (int64_t)nb,
+        [&](int64_t b0, int64_t b1) {
+          for (int64_t b = b0; b < b1; ++b) {
+            for (int c = 0; c < 3; ++c) base[3 * b + c] = in.density.blocks[b].base[c];
+            std::memcpy(&dens[64 * b], in.density.blocks[b].data, 64 * sizeof(double));
+          }
+        },
+        1024);
     std::vector<int32_t> dc(in.dirichlet_nodes.size() * 3), da(in.dirichlet_nodes.size());
     std::vector<double> dv(in.dirichlet_nodes.size());
     for (size_t i = 0; i < in.dirichlet_nodes.size(); ++i) {
@@ -378,12 +410,15 @@ void fem_solve(void* input, void* output) {
     // blocks, Dirichlet data and settings are the same and no cell switches
     // on or off; otherwise the solver is rebuilt (the reference re-flags
     // every call, :102-145) and the previous u is carried over by block.
+    trace("unpack");
     const bool warm = h != nullptr;
     WarmState prev;
     if (h) prev = std::move(h->warm);
     bool rebuilt = true;
     if (h && same_topology(h.get(), in, base) && h->s->active_set_matches(dens.data())) {
+      trace("reuse checks");
       h->s->update_density(dens.data());
+      trace("update density");
       rebuilt = false;
     } else {
       h.reset();
@@ -392,6 +427,7 @@ void fem_solve(void* input, void* output) {
                           dv.data(), &nh);
       if (rc != SPG_OK) throw SpgError(rc, g_err);
       h.reset(nh);
+      trace("create solver");
     }
     if (warm && rebuilt && prev.valid) {
       const std::vector<double> g = carry_over(prev, base);
@@ -401,6 +437,7 @@ void fem_solve(void* input, void* output) {
       if (hipMemcpy(gd, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         throw SpgError(SPG_E_HIP, "hipMemcpy failed");
       h->s->set_initial_guess_in3(gd);
+      trace("carry over");
     }
     std::vector<int32_t> fc(in.forces.size() * 3);
     std::vector<double> fv(in.forces.size() * 3);
@@ -410,6 +447,7 @@ void fem_solve(void* input, void* output) {
         fv[3 * i + c] = in.forces[i].f[c];
       }
     h->s->set_forces((int64_t)in.forces.size(), fc.data(), fv.data());
+    trace("forces");
     spg_solve_params prm{};
     prm.tolerance = in.krylov.tolerance;
     prm.max_iterations = in.krylov.max_iterations > 0 ? in.krylov.max_iterations : 500;
@@ -423,13 +461,7 @@ void fem_solve(void* input, void* output) {
     prm.reference_linf = warm && prev.valid ? prev.initial_linf : 0.0;
     spg_info info{};
     const int rc = h->s->solve(prm, &info);
-    out.displacements.blocks.resize(nb);
-    std::vector<double> packed(nb * 64 * 4);
-    h->s->pack_output(packed.data());
-    for (size_t b = 0; b < nb; ++b) {
-      for (int c = 0; c < 3; ++c) out.displacements.blocks[b].base[c] = base[3 * b + c];
-      std::memcpy(out.displacements.blocks[b].data, &packed[b * 256], 256 * sizeof(double));
-    }
+    trace("solve");
     out.residuals.push_back(info.r_linf / (info.r0_linf > 0 ? info.r0_linf : 1.0));
     out.error_code = rc;
     out.success = (rc == SPG_OK);
@@ -437,19 +469,42 @@ void fem_solve(void* input, void* output) {
                " r0_linf=" + std::to_string(info.r0_linf) + " r_linf=" + std::to_string(info.r_linf) +
                (warm ? (info.warm_reused ? " warm_start=reused" : " warm_start=discarded") : "");
     if (rc != SPG_OK) out.info += " (not converged or failed)";
+    if (keep) out.set_state_ptr(h.get());
+    // the stream in one allocation: head, the displacement block records
+    // (packed on the device and copied into place once), tail
+    WireWriter head, tail(false);
+    write_outputs_head(head, out, nb);
+    write_outputs_tail(tail, out);
+    const size_t hb = head.bytes().size(), rb = nb * kRecordBytes, tb = tail.bytes().size();
+    std::unique_ptr<char[]> data(new char[hb + rb + tb]);
+    uint8_t* recs = reinterpret_cast<uint8_t*>(data.get()) + hb;
+    h->s->pack_output_records(base.data(), recs);
+    const uint64_t total = hb + rb + tb;
+    std::memcpy(data.get(), head.bytes().data(), hb);
+    std::memcpy(data.get(), &total, sizeof(total));
+    std::memcpy(recs + rb, tail.bytes().data(), tb);
+    trace("pack output");
     // what the next warm start begins from: the displacement of a finished
     // solve, and the first solve's zero-guess residual
     if (rc == SPG_OK || rc == SPG_E_NOT_CONVERGED) {
       WarmState& w = h->warm;
       w.base = base;
       w.u.resize(nb * 192);
-      for (size_t i = 0; i < nb * 64; ++i)
-        for (int c = 0; c < 3; ++c) w.u[i * 3 + c] = packed[i * 4 + c];
+      parallel_ranges(
+          (int64_t)nb,
+          [&](int64_t b0, int64_t b1) {
+            for (int64_t b = b0; b < b1; ++b)
+              for (int l = 0; l < 64; ++l)
+                std::memcpy(&w.u[(b * 64 + l) * 3], recs + b * kRecordBytes + 12 + l * 32, 3 * sizeof(double));
+          },
+          256);
       w.initial_linf = prev.valid ? prev.initial_linf : info.r0_linf;
       w.valid = true;
     } else {
       h->warm = std::move(prev);
     }
+    emitted = std::move(data);
+    trace("keep warm state");
   } catch (SpgError& e) {
     out.success = false;
     out.error_code = e.code;
@@ -473,11 +528,17 @@ void fem_solve(void* input, void* output) {
   }
   if (keep && h) out.set_state_ptr(h.release());
   h.reset();
+  trace("release");
+  if (emitted) {
+    *reinterpret_cast<void**>(output) = emitted.release();
+    return;
+  }
   try {
     emit(out, output);
   } catch (...) {
     *reinterpret_cast<void**>(output) = nullptr;
   }
+  trace("write outputs");
 }
 
 // The reference interface test's fake solver (topo_opt/interface_test.cpp:7-34):

@@ -137,7 +137,10 @@ class WireReader {
 // ------------------------------------------------------------------ writer
 class WireWriter {
  public:
-  WireWriter() { buf_.resize(sizeof(uint64_t), 0); }
+  // with `length_header` the stream opens with its total length (finalize)
+  explicit WireWriter(bool length_header = true) {
+    if (length_header) buf_.resize(sizeof(uint64_t), 0);
+  }
   template <class T> void pod(const T& v) {
     size_t h = buf_.size();
     buf_.resize(h + sizeof(T));
@@ -152,6 +155,7 @@ class WireWriter {
     pod<uint64_t>(s.size());
     raw(s.data(), s.size());
   }
+  const std::vector<uint8_t>& bytes() const { return buf_; }
   std::vector<uint8_t>& finalize() {
     uint64_t n = buf_.size();
     std::memcpy(&buf_[0], &n, sizeof(uint64_t));
@@ -298,15 +302,25 @@ inline void write_inputs(WireWriter& w, const FemInputs& in) {
   w.str(in.caller_method);
 }
 
-inline void write_outputs(WireWriter& w, const FemOutputs& o) {
+// FEMOutputs = head, `nblocks` displacement block records (int32 base[3],
+// then 64 nodes x 4 doubles), tail
+inline void write_outputs_head(WireWriter& w, const FemOutputs& o, uint64_t nblocks) {
   write_base(w, o);
-  write_vector_grid(w, o.displacements);
+  w.raw(o.displacements.resolution, 12);
+  w.pod<uint64_t>(nblocks);
+}
+inline void write_outputs_tail(WireWriter& w, const FemOutputs& o) {
   write_scalar_grid(w, o.residual_field);
   w.str(o.info);
   w.pod<uint64_t>(o.residuals.size());
   for (double d : o.residuals) w.pod(d);
   w.pod(o.error_code);
   w.pod<uint8_t>(o.success ? 1 : 0);
+}
+inline void write_outputs(WireWriter& w, const FemOutputs& o) {
+  write_outputs_head(w, o, o.displacements.blocks.size());
+  for (auto& b : o.displacements.blocks) { w.raw(b.base, 12); w.raw(b.data, 64 * 32); }
+  write_outputs_tail(w, o);
 }
 
 inline void read_outputs(WireReader& r, FemOutputs& o) {

@@ -2634,6 +2634,36 @@ __global__ void k_gather_input_vec(const int* in_slot, int64_t nin, const double
   const int blk = s >> 6, l = s & 63;
   for (int c = 0; c < 3; ++c) out3[i * 3 + c] = s >= 0 ? v[vidx(blk, c, l)] : 0.0;
 }
+// Output records, one per input block: `hdr` 32-bit words of header (the
+// block's base coordinates when hdr = 3), then its 64 nodes as
+// {u_x, u_y, u_z, max_c |r_c|} doubles, zero for a node without a slot
+// (SPGrid_CPU_Solver.cpp:345-384).  Stored as 32-bit words: after a 12-byte
+// header the doubles are only 4-byte aligned.
+__global__ void k_pack_out4(const int* __restrict__ in_slot, int64_t nin, const double* __restrict__ u,
+                            const double* __restrict__ r, const int32_t* __restrict__ base, int hdr,
+                            uint32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nin) return;
+  const int64_t b = i >> 6;
+  const int l = (int)(i & 63);
+  uint32_t* rec = out + b * (hdr + 512);
+  if (l < hdr) rec[l] = (uint32_t)base[b * 3 + l];
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  const int s = in_slot[i];
+  if (s >= 0) {
+    const int blk = s >> 6, sl = s & 63;
+    for (int c = 0; c < 3; ++c) {
+      o[c] = u[vidx(blk, c, sl)];
+      o[3] = fmax(o[3], fabs(r[vidx(blk, c, sl)]));
+    }
+  }
+  uint32_t* dst = rec + hdr + l * 8;
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t w = (uint64_t)__double_as_longlong(o[k]);
+    dst[2 * k] = (uint32_t)w;
+    dst[2 * k + 1] = (uint32_t)(w >> 32);
+  }
+}
 __global__ void k_gather_input_scalar(const int* in_slot, int64_t nin, const double* v, double* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nin) return;
@@ -3215,6 +3245,10 @@ void launch_scatter_input_vec(hipStream_t st, const int* in_slot, int64_t nin, c
 }
 void launch_gather_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out3) {
   if (nin > 0) k_gather_input_vec<<<nblk(nin), TPB, 0, st>>>(in_slot, nin, v, out3);
+}
+void launch_pack_out4(hipStream_t st, const int* in_slot, int64_t nin, const double* u, const double* r,
+                      const int32_t* base, int hdr, uint32_t* out) {
+  if (nin > 0) k_pack_out4<<<nblk(nin), TPB, 0, st>>>(in_slot, nin, u, r, base, hdr, out);
 }
 void launch_gather_input_scalar(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out) {
   if (nin > 0) k_gather_input_scalar<<<nblk(nin), TPB, 0, st>>>(in_slot, nin, v, out);

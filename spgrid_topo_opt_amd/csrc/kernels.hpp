@@ -225,6 +225,11 @@ void launch_sensitivity(hipStream_t st, const LevelDev& L, const double* rho, co
 void launch_scatter_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* in3, double* v);
 void launch_gather_input_vec(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out3);
 void launch_gather_input_scalar(hipStream_t st, const int* in_slot, int64_t nin, const double* v, double* out);
+// per input block (nin = 64 x blocks): `hdr` int32 words of base
+// coordinates (0 or 3, from base[3b..3b+2]) and 64 x {u_x, u_y, u_z,
+// max_c |r_c|} doubles, as 32-bit words
+void launch_pack_out4(hipStream_t st, const int* in_slot, int64_t nin, const double* u, const double* r,
+                      const int32_t* base, int hdr, uint32_t* out);
 // v[idx[i]] = val[i] (entry = slot * 3 + component) in parallel: the entries must be distinct
 void launch_set_entries(hipStream_t st, double* v, const int* idx, const double* val, int64_t n);
 

@@ -118,34 +118,6 @@ void HostLevel::mark_page(int x, int y, int z) {
 
 static inline int loc(int x, int y, int z) { return ((x & 3) << 4) | ((y & 3) << 2) | (z & 3); }
 
-// f(begin, end) over [0, n) on up to 16 threads (blocks are independent in
-// every loop that uses it); the first exception is rethrown after the join
-template <class F>
-static void parallel_ranges(int64_t n, F&& f, int64_t min_chunk = 4096) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::min(hw, 16u)), (n + min_chunk - 1) / min_chunk);
-  if (nt <= 1) {
-    f((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  std::vector<std::exception_ptr> err(nt);
-  const int64_t chunk = (n + nt - 1) / nt;
-  for (int64_t t = 0; t < nt; ++t) {
-    const int64_t a = t * chunk, b = std::min(n, a + chunk);
-    th.emplace_back([&, t, a, b] {
-      try {
-        if (a < b) f(a, b);
-      } catch (...) {
-        err[t] = std::current_exception();
-      }
-    });
-  }
-  for (auto& x : th) x.join();
-  for (auto& e : err)
-    if (e) std::rethrow_exception(e);
-}
-
 // Flag_Nodes (Flaging.h:127-142): every node takes the ACTIVE / DIRICHLET
 // bits (>>4) of the cells of allocated blocks that have it as a corner.  The
 // blocks that receive bits are created first (serially: a cell reaches the +

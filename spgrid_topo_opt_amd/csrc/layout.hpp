@@ -11,16 +11,47 @@
 // Only blocks holding at least one ACTIVE node are shipped to the GPU: every
 // other node carries zeros in all solver vectors.
 #pragma once
+#include <algorithm>
 #include <cstdint>
+#include <exception>
 #include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 #include <vector>
 
 namespace spg {
+
+// f(begin, end) over [0, n) on up to 16 threads (the ranges must be
+// independent); the exception of the lowest range is rethrown after the join
+template <class F>
+inline void parallel_ranges(int64_t n, F&& f, int64_t min_chunk = 4096) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int64_t nt = std::min<int64_t>(std::max<unsigned>(1u, std::min(hw, 16u)), (n + min_chunk - 1) / min_chunk);
+  if (nt <= 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nt);
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t a = t * chunk, b = std::min(n, a + chunk);
+    th.emplace_back([&, t, a, b] {
+      try {
+        if (a < b) f(a, b);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
 
 // reference flag bits (solver/Flags.h:10-24)
 constexpr uint16_t F_ACTIVE_NODE = 0x0010;

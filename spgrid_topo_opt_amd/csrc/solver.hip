@@ -510,48 +510,77 @@ void Solver::build_boundary_rows() {
   sync();
 }
 
+// The lowest i in [0, n) with ev(i) != 0, and ev there (n and 0 if none):
+// the answer of one pass in order, found on up to 16 threads
+template <class F>
+static std::pair<int64_t, int> first_event(int64_t n, F&& ev) {
+  std::mutex m;
+  int64_t at = n;
+  int kind = 0;
+  parallel_ranges(
+      n,
+      [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i)
+          if (const int k = ev(i)) {
+            std::lock_guard<std::mutex> g(m);
+            if (i < at) {
+              at = i;
+              kind = k;
+            }
+            return;
+          }
+      },
+      1 << 15);
+  return {at, kind};
+}
+
 bool Solver::active_set_matches(const double* density) const {
   const Layout& Lc = lay_;
-  for (int64_t b = 0; b < Lc.nin_blocks; ++b)
-    for (int l = 0; l < 64; ++l) {
-      const int32_t s = Lc.in_node_slot[b * 64 + l];
-      const double d = density[b * 64 + l];
-      if (std::isnan(d)) throw SpgError(SPG_E_NAN, "density contains NaN");
-      if (s < 0) {
-        // a cell whose min corner has no GPU slot is inactive; it may not turn on
-        if (d > 0) return false;
-        continue;
-      }
-      if ((Lc.E[s] != 0.0) != (d > 0)) return false;
-    }
-  return true;
+  // 1: NaN (throws), 2: a cell would switch on or off
+  const int kind = first_event(Lc.nin_blocks * 64, [&](int64_t i) {
+    const int32_t s = Lc.in_node_slot[i];
+    const double d = density[i];
+    if (std::isnan(d)) return 1;
+    // a cell whose min corner has no GPU slot is inactive; it may not turn on
+    if (s < 0) return d > 0 ? 2 : 0;
+    return (Lc.E[s] != 0.0) != (d > 0) ? 2 : 0;
+  }).second;
+  if (kind == 1) throw SpgError(SPG_E_NAN, "density contains NaN");
+  return kind == 0;
 }
 
 void Solver::update_density(const double* density) {
   activate();
-  // recompute E (SIMP) for the existing cells, then the coarse operators
-  const Layout& Lc = lay_;
-  std::vector<double> E(Lc.E.size(), 0.0), rho(Lc.rho.size(), 0.0);
-  for (int64_t b = 0; b < Lc.nin_blocks; ++b)
-    for (int l = 0; l < 64; ++l) {
-      const int32_t s = Lc.in_node_slot[b * 64 + l];
-      if (s < 0) continue;
-      const double d = density[b * 64 + l];
-      if (std::isnan(d)) throw SpgError(SPG_E_NAN, "density contains NaN");
-      rho[s] = d;
-      if (Lc.E[s] != 0.0) {
-        if (!(d > 0)) throw SpgError(SPG_E_INVALID, "density update would deactivate a cell (topology change)");
-        E[s] = stiffness_mapping(cfg_.penalty, cfg_.min_stiffness, cfg_.min_density, d);
-      } else if (d > 0) {
-        throw SpgError(SPG_E_INVALID, "density update would activate a cell (topology change)");
-      }
-    }
-  lay_.E.assign(E.begin(), E.end());
-  lay_.rho.assign(rho.begin(), rho.end());
-  HIPCHK(hipMemcpy(E64_.p, E.data(), E.size() * 8, hipMemcpyHostToDevice));
-  std::vector<float> e32(E.begin(), E.end());
-  HIPCHK(hipMemcpy(E32_.p, e32.data(), e32.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(rho_.p, rho.data(), rho.size() * 8, hipMemcpyHostToDevice));
+  // recompute E (SIMP) for the existing cells, then the coarse operators;
+  // nothing changes when the check fails
+  const int64_t nin = lay_.nin_blocks * 64;
+  const int kind = first_event(nin, [&](int64_t i) {
+    const int32_t s = lay_.in_node_slot[i];
+    if (s < 0) return 0;
+    const double d = density[i];
+    if (std::isnan(d)) return 1;
+    if (lay_.E[s] != 0.0) return d > 0 ? 0 : 2;
+    return d > 0 ? 3 : 0;
+  }).second;
+  if (kind == 1) throw SpgError(SPG_E_NAN, "density contains NaN");
+  if (kind == 2) throw SpgError(SPG_E_INVALID, "density update would deactivate a cell (topology change)");
+  if (kind == 3) throw SpgError(SPG_E_INVALID, "density update would activate a cell (topology change)");
+  // input nodes have distinct slots: the ranges write disjoint entries
+  parallel_ranges(
+      nin,
+      [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i) {
+          const int32_t s = lay_.in_node_slot[i];
+          if (s < 0) continue;
+          const double d = density[i];
+          lay_.rho[s] = d;
+          if (lay_.E[s] != 0.0) lay_.E[s] = stiffness_mapping(cfg_.penalty, cfg_.min_stiffness, cfg_.min_density, d);
+        }
+      },
+      1 << 15);
+  HIPCHK(hipMemcpy(E64_.p, lay_.E.data(), lay_.E.size() * 8, hipMemcpyHostToDevice));
+  launch_cast_d2f(st_, E64_.as<double>(), E32_.as<float>(), (int64_t)lay_.E.size());
+  HIPCHK(hipMemcpy(rho_.p, lay_.rho.data(), lay_.rho.size() * 8, hipMemcpyHostToDevice));
   build_operators();
 }
 
@@ -1049,32 +1078,31 @@ int Solver::solve_adjoint(const spg_solve_params& prm, const double* g_in3, doub
   return rc;
 }
 
-void Solver::pack_output(double* out4) const {
-  const int64_t n = lv_[0].n3;
-  std::vector<double> u(n), r(n);
-  HIPCHK(hipMemcpy(u.data(), u_.p, n * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(r.data(), res_.p, n * 8, hipMemcpyDeviceToHost));
-  const int64_t nin = lay_.nin_blocks * 64;
-  for (int64_t i = 0; i < nin; ++i) {
-    const int32_t s = lay_.in_node_slot[i];
-    double* o = out4 + i * 4;
-    if (s >= 0) {
-      const int blk = s >> 6, l = s & 63;
-      double m = 0;
-      for (int c = 0; c < 3; ++c) {
-        o[c] = u[(blk * 3 + c) * 64 + l];
-        m = std::max(m, std::fabs(r[(blk * 3 + c) * 64 + l]));
-      }
-      o[3] = m;
-    } else {
-      o[0] = o[1] = o[2] = o[3] = 0.0;
+void Solver::pack_records(const int32_t* base, int hdr, uint8_t* dst) const {
+  const int64_t nb = lay_.nin_blocks, nin = nb * 64;
+  const size_t rec = (size_t)hdr * 4 + 64 * 32;
+  if (nb > 0) {
+    DBuf out, bdev;
+    out.alloc((size_t)nb * rec);
+    if (hdr > 0) {
+      bdev.alloc((size_t)nb * 3 * sizeof(int32_t));
+      HIPCHK(hipMemcpyAsync(bdev.p, base, (size_t)nb * 3 * sizeof(int32_t), hipMemcpyHostToDevice, st_));
     }
+    launch_pack_out4(st_, in_slot_.as<int>(), nin, u_.as<double>(), res_.as<double>(), bdev.as<int32_t>(), hdr,
+                     out.as<uint32_t>());
+    HIPCHK(hipStreamSynchronize(st_));
+    HIPCHK(hipMemcpy(dst, out.p, (size_t)nb * rec, hipMemcpyDeviceToHost));
   }
+  // Dirichlet entries of nodes without a slot keep their prescribed value
   for (size_t k = 0; k < lay_.ud_extra.size(); ++k) {
-    const int64_t e = lay_.ud_extra[k];
-    out4[(e / 3) * 4 + (e % 3)] = lay_.ud_extra_value[k];
+    const int64_t e = lay_.ud_extra[k], n = e / 3;
+    std::memcpy(dst + (n >> 6) * rec + hdr * 4 + ((n & 63) * 4 + e % 3) * 8, &lay_.ud_extra_value[k], 8);
   }
 }
+
+void Solver::pack_output(double* out4) const { pack_records(nullptr, 0, reinterpret_cast<uint8_t*>(out4)); }
+
+void Solver::pack_output_records(const int32_t* base, uint8_t* dst) const { pack_records(base, 3, dst); }
 
 void Solver::displacement_in3(double* dev) const {
   launch_gather_input_vec(st_, in_slot_.as<int>(), lay_.nin_blocks * 64, u_.as<double>(), dev);

@@ -78,7 +78,10 @@ class Solver {
   // returns SPG_OK or error code; fills info
   int solve(const spg_solve_params& prm, spg_info* info);
   int solve_adjoint(const spg_solve_params& prm, const double* g_dev_in3, double* lam_dev_in3, spg_info* info);
-  void pack_output(double* out4) const;  // host
+  void pack_output(double* out4) const;  // host, [nin_blocks * 64][4]
+  // host, per input block the FEMOutputs displacement record: int32
+  // base[3] (from base[3b..3b+2]) then 64 x 4 doubles (2060 bytes)
+  void pack_output_records(const int32_t* base, uint8_t* dst) const;
   void displacement_in3(double* dev) const;
   // starting guess of the next warm-started solve (input order, device)
   void set_initial_guess_in3(const double* dev);
@@ -101,6 +104,7 @@ class Solver {
   // z = M b0; `z_is_zero` says the caller has already cleared z
   template <class T> void vcycle(const T* b0, T* z, bool z_is_zero = false);
   void vcycle_coarse64(const float* b0, float* z, bool z_is_zero);
+  void pack_records(const int32_t* base, int hdr, uint8_t* dst) const;
   template <class T> void smooth(int l, T* u, const T* b, bool forward, const T* bcm = nullptr);
   // the level-0 colour sweeps read b and the flags from colour-major copies
   // when the level's working set exceeds the memory-side cache (or
