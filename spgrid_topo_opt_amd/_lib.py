@@ -362,7 +362,8 @@ def oc_trial_noalloc(n, mask_ptr, rho_ptr, dc_ptr, mid, step, min_density, resul
 def fem_solve_bytes(data: bytes, test=False) -> bytes:
     """Call the C entry point `fem_solve` (or `fem_solve_test`) on a serialized
     FEMInputs stream and return the serialized FEMOutputs stream."""
-    buf = C.create_string_buffer(data, len(data))
+    data = bytes(data)
+    buf = C.c_char_p(data)  # the bytes object's own storage: read, not copied
     out = C.c_void_p()
     fn = lib().fem_solve_test if test else lib().fem_solve
     fn(C.cast(buf, C.c_void_p), C.byref(out))

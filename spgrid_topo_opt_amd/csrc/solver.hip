@@ -130,6 +130,7 @@ Solver::Solver(const spg_config& cfg, const ProblemDesc& pd) : cfg_(cfg) {
 Solver::~Solver() {
   if (rb_) rocblas_destroy_handle(rb_);
   if (hscal_) (void)hipHostFree(hscal_);
+  if (stage_) (void)hipHostFree(stage_);
   if (ev_a_) (void)hipEventDestroy(ev_a_);
   if (ev_b_) (void)hipEventDestroy(ev_b_);
   if (st2_) (void)hipStreamDestroy(st2_);
@@ -1090,8 +1091,21 @@ void Solver::pack_records(const int32_t* base, int hdr, uint8_t* dst) const {
     }
     launch_pack_out4(st_, in_slot_.as<int>(), nin, u_.as<double>(), res_.as<double>(), bdev.as<int32_t>(), hdr,
                      out.as<uint32_t>());
+    // through a kept pinned buffer, copied out on host threads: a pageable
+    // copy of this size runs at a few GB/s on one thread
+    const size_t bytes = (size_t)nb * rec;
+    if (stage_bytes_ < bytes) {
+      if (stage_) (void)hipHostFree(stage_);
+      stage_ = nullptr;
+      stage_bytes_ = 0;
+      HIPCHK(hipHostMalloc(&stage_, bytes));
+      stage_bytes_ = bytes;
+    }
+    HIPCHK(hipMemcpyAsync(stage_, out.p, bytes, hipMemcpyDeviceToHost, st_));
     HIPCHK(hipStreamSynchronize(st_));
-    HIPCHK(hipMemcpy(dst, out.p, (size_t)nb * rec, hipMemcpyDeviceToHost));
+    const uint8_t* src = static_cast<const uint8_t*>(stage_);
+    parallel_ranges(
+        (int64_t)bytes, [&](int64_t a, int64_t b) { std::memcpy(dst + a, src + a, (size_t)(b - a)); }, 1 << 22);
   }
   // Dirichlet entries of nodes without a slot keep their prescribed value
   for (size_t k = 0; k < lay_.ud_extra.size(); ++k) {

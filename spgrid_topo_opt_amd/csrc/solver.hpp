@@ -164,6 +164,8 @@ class Solver {
   DBuf apart_;  // per-workgroup partials of the fused operator apply + dot
   DBuf fcm_, bcm_;  // level-0 flags and b in colour-major order (either precision)
   double* hscal_ = nullptr;
+  mutable void* stage_ = nullptr;  // pinned staging of the output records
+  mutable size_t stage_bytes_ = 0;
   double setup_s_ = 0.0;
   double dev_bytes_ = 0.0;
   const double* posmat_device(int depth);
