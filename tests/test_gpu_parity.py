@@ -678,6 +678,40 @@ def test_fem_solve_end_to_end(box3):
     assert outp.residuals and outp.residuals[-1] <= 1e-10
 
 
+def test_fem_solve_blocks_off_the_4_grid():
+    """Input blocks whose x bases are 1 mod 4, plus a layer of empty blocks
+    at the end: the nodes of that layer with local x = 3 fall outside every
+    allocated page.  A Dirichlet value given at one of them appears only in
+    its output entry (SPGrid_CPU_Solver.cpp:241-246).  The whole output
+    matches the oracle's."""
+    p = W.box(16, 8, 8, seed=3, mg_levels=3)
+    sh = np.array([1, 0, 0], np.int32)
+    extra = p.block_base[p.block_base[:, 0] == p.block_base[:, 0].max()] + np.array([4, 0, 0], np.int32)
+    bb = (np.concatenate([p.block_base, extra]) + sh).astype(np.int32)
+    node = extra[0] + sh + np.array([3, 1, 2], np.int32)
+    q = W.Problem(**{**p.__dict__, "name": "box16x8x8 shifted", "block_base": bb,
+                     "density": np.concatenate([p.density, np.zeros((len(extra), 64))]),
+                     "dir_coord": np.concatenate([p.dir_coord + sh, node[None]]).astype(np.int32),
+                     "dir_axis": np.concatenate([p.dir_axis, [1]]).astype(np.int32),
+                     "dir_value": np.concatenate([p.dir_value, [0.25]]),
+                     "force_coord": (p.force_coord + sh).astype(np.int32)})
+    o = O.Oracle(q)
+    u_o, _, _ = o.solve(tolerance=1e-10, max_iterations=500)
+    want = o.pack(u_o)
+    outp = FI.FEMInterface(FI.FEMInputs.from_problem(q, tolerance=1e-10, max_iterations=500,
+                                                     extra_ints={"defect_correction_iter": 20},
+                                                     extra_doubles={"inner_tolerance": 1e-3})).run()
+    assert outp.success, outp.info
+    assert (outp.displacement_base == bb).all()
+    got = outp.displacements
+    bi = int(np.flatnonzero((bb == node - np.array([3, 1, 2])).all(1))[0])
+    l = 3 * 16 + 1 * 4 + 2
+    assert got[bi, l].tolist() == [0.0, 0.25, 0.0, 0.0]
+    assert want[bi, l].tolist() == [0.0, 0.25, 0.0, 0.0]
+    err = np.linalg.norm(got[..., :3] - want[..., :3]) / np.linalg.norm(want[..., :3])
+    assert err < 1e-8, err
+
+
 def test_sensitivity_matches_oracle(box3):
     torch = pytest.importorskip("torch")
     P = box3
