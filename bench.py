@@ -75,7 +75,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md and not implemented.", file=sys.stderr)
+              "sharding the block set over several GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
     if args.opt_loop:
         opt_loop(args)
