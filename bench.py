@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the FP32 apply and a 512 MiB copy (5 launches each) for rocprofv3 --pmc passes")
     ap.add_argument("--opt-loop", action="store_true", help="time the OC optimization loop on the 512x256x256 band")
+    ap.add_argument("--fem-solve-calls", type=int, default=0,
+                    help="time N calls of the drop-in fem_solve on the shell (wire format in and out, keep_state; "
+                         "call k > 0 scales the densities by 1 - 0.02 k); SPG_SETUP_TRACE=2 prints the stages")
     ap.add_argument("--opt-iters", type=int, default=20)
     ap.add_argument("--opt-precisions", default="mixed,fp64")
     ap.add_argument("--opt-band-start", type=int, default=5)
@@ -79,6 +82,9 @@ def main():
         sys.exit(2)
     if args.opt_loop:
         opt_loop(args)
+        return
+    if args.fem_solve_calls > 0:
+        fem_solve_calls(args)
         return
 
     import torch
@@ -416,6 +422,53 @@ def traffic_probe(s, prob):
         y.copy_(x)
     torch.cuda.synchronize()
     log(f"[bench] traffic probe done: FP32 apply x5+1, GS sweep, boundary phases, copy of {COPY_BYTES} B x3")
+
+
+def fem_solve_calls(args):
+    """The reference caller's path: FEMInputs serialized, `fem_solve`, FEMOutputs
+    parsed, with the solver kept between calls (keep_state; the last call
+    releases it).  Densities change between calls within the same active set,
+    so every call after the first updates the kept hierarchy and re-solves."""
+    import dataclasses
+
+    from spgrid_topo_opt_amd import _lib
+    from spgrid_topo_opt_amd import fem_interface as FI
+    from spgrid_topo_opt_amd import workloads as W
+
+    t0 = time.perf_counter()
+    prob = W.config("cfg2", n=args.n, mg_levels=args.levels)
+    log(f"[wire] problem {prob.name}: {prob.block_base.shape[0]} blocks, built in {time.perf_counter() - t0:.1f}s")
+    calls, state = [], None
+    for k in range(args.fem_solve_calls):
+        last = k + 1 == args.fem_solve_calls
+        p = dataclasses.replace(prob, density=prob.density * (1.0 - 0.02 * k))
+        inp = FI.FEMInputs.from_problem(p, tolerance=args.tol, max_iterations=args.max_iterations,
+                                        extra_ints={"keep_state": 0 if last else 1})
+        if state:
+            inp.set_solver_state_ptr(state)
+        t0 = time.perf_counter()
+        data = inp.serialize()
+        t1 = time.perf_counter()
+        raw = _lib.fem_solve_bytes(data)
+        t2 = time.perf_counter()
+        out = FI.FEMOutputs.deserialize(raw)
+        t3 = time.perf_counter()
+        state = out.get_solver_state_ptr()
+        rec = {"serialize_s": t1 - t0, "fem_solve_s": t2 - t1, "deserialize_s": t3 - t2, "in_mb": len(data) / 1e6,
+               "out_mb": len(raw) / 1e6, "success": bool(out.success), "info": out.info}
+        calls.append(rec)
+        log(f"[wire] call {k}: serialize {rec['serialize_s']:.3f}s fem_solve {rec['fem_solve_s']:.3f}s "
+            f"deserialize {rec['deserialize_s']:.3f}s {out.info}")
+        if not out.success:
+            raise SystemExit(f"fem_solve failed: {out.info}")
+    warm = [c["fem_solve_s"] for c in calls[1:]]
+    print(json.dumps({
+        "metric": "seconds per fem_solve call (wire format in and out, kept solver, new densities)",
+        "value": sorted(warm)[len(warm) // 2] if warm else calls[0]["fem_solve_s"],
+        "unit": "s", "n_gpus": 1, "steps": len(calls), "higher_is_better": False, "data": "synthetic",
+        "config": {"workload": f"sphere shell {args.n}^3", "mg_levels": args.levels, "tolerance": args.tol,
+                   "blocks": int(prob.block_base.shape[0])},
+        "first_call_s": calls[0]["fem_solve_s"], "calls": calls}), flush=True)
 
 
 def opt_loop(args):
