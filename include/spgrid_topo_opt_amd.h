@@ -43,7 +43,10 @@ extern "C" {
  * (topo_opt/fem_interface.h:11; defined in solver/SPGrid_CPU_Solver.cpp:26).
  * `input` points at a serialized FEMInputs stream; `output` is really a
  * `void**` that receives a new[]-allocated serialized FEMOutputs.  Failures
- * are reported in FEMOutputs.success / error_code / info.                    */
+ * are reported in FEMOutputs.success / error_code / info.  A solve that ran
+ * to its end reports success = 1 even when the tolerance was not reached
+ * (error_code SPG_E_NOT_CONVERGED), as the reference does
+ * (SPGrid_CPU_Solver.cpp:368).                                               */
 void fem_solve(void *input, void *output);
 
 /* Fake solver used by the reference's interface test (u_p = rho + p):
@@ -66,6 +69,12 @@ typedef struct {
   int32_t smooth_iters;    /* FEMInputs::pre_and_post_smoothing_iter         */
   int32_t boundary_iters;  /* extra_ints["boundary_smoothing"]              */
   int32_t device;          /* HIP device ordinal                            */
+  int32_t bottom_dense_max; /* largest bottom level (unknowns) solved with
+                              the dense FP64 inverse; larger bottoms use the
+                              nested-dissection Cholesky.  0: 16384, the
+                              choice for many solves per factorization;
+                              fem_solve, which factors once per call, uses
+                              4096.  At most 60000.                         */
 } spg_config;
 
 /* Solve parameters (krylov + defect-correction knobs of FEMInputs). */

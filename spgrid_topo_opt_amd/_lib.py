@@ -56,6 +56,7 @@ class Config(C.Structure):
         ("smooth_iters", C.c_int32),
         ("boundary_iters", C.c_int32),
         ("device", C.c_int32),
+        ("bottom_dense_max", C.c_int32),
     ]
 
 
@@ -192,9 +193,12 @@ def make_config(prob, device=0):
 class Solver:
     """GPU solver for one problem (spgrid_topo_opt_amd.workloads.Problem)."""
 
-    def __init__(self, prob, device=0):
+    def __init__(self, prob, device=0, bottom_dense_max=0):
+        """bottom_dense_max: the largest bottom (unknowns) factored as a dense
+        inverse, 0 for the library's default (spg_config.bottom_dense_max)"""
         self.prob = prob
         self.cfg = make_config(prob, device)
+        self.cfg.bottom_dense_max = int(bottom_dense_max)
         bb = np.ascontiguousarray(prob.block_base, np.int32)
         de = np.ascontiguousarray(prob.density, np.float64)
         dc = np.ascontiguousarray(prob.dir_coord, np.int32).reshape(-1, 3)

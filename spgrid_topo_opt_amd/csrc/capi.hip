@@ -41,6 +41,9 @@ struct spg_solver {
 
 static thread_local std::string g_err;
 
+// largest bottom (unknowns) fem_solve factors as a dense inverse
+constexpr int32_t kFemSolveDenseBottomDofs = 4096;
+
 // a displacement block of the FEMOutputs stream: int32 base[3], 64 x 4 doubles
 constexpr size_t kRecordBytes = 12 + 64 * 32;
 
@@ -405,6 +408,9 @@ void fem_solve(void* input, void* output) {
     cfg.smooth_iters = in.pre_and_post_smoothing_iter;
     cfg.boundary_iters = in.get_int("boundary_smoothing", 5);
     cfg.device = in.get_int("device", 0);
+    // a call factors the bottom once and solves once: the nested-dissection
+    // Cholesky factors faster than the dense inverse above about 4 K unknowns
+    cfg.bottom_dense_max = kFemSolveDenseBottomDofs;
     // solver state reuse (fem_interface.h:193-209, SPGrid_CPU_Solver.cpp:58-75):
     // a state pointer means a warm start.  The kept hierarchy serves when the
     // blocks, Dirichlet data and settings are the same and no cell switches
@@ -463,12 +469,17 @@ void fem_solve(void* input, void* output) {
     const int rc = h->s->solve(prm, &info);
     trace("solve");
     out.residuals.push_back(info.r_linf / (info.r0_linf > 0 ? info.r0_linf : 1.0));
+    // a solve that ran to its end succeeds, converged or not, as the
+    // reference's does (SPGrid_CPU_Solver.cpp:326,368): its caller stops the
+    // optimization on success = 0 (objective/minimal_compliance.cpp:64-67).
+    // error_code and info still say when the tolerance was not met.
     out.error_code = rc;
-    out.success = (rc == SPG_OK);
+    out.success = (rc == SPG_OK || rc == SPG_E_NOT_CONVERGED);
     out.info = "iterations=" + std::to_string(info.iterations) + " passes=" + std::to_string(info.passes) +
                " r0_linf=" + std::to_string(info.r0_linf) + " r_linf=" + std::to_string(info.r_linf) +
                (warm ? (info.warm_reused ? " warm_start=reused" : " warm_start=discarded") : "");
-    if (rc != SPG_OK) out.info += " (not converged or failed)";
+    if (rc == SPG_E_NOT_CONVERGED) out.info += " (tolerance not reached)";
+    else if (rc != SPG_OK) out.info += " (failed)";
     if (keep) out.set_state_ptr(h.get());
     // the stream in one allocation: head, the displacement block records
     // (packed on the device and copied into place once), tail

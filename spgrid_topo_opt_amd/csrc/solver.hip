@@ -44,8 +44,13 @@ static constexpr int kCoopWordOffset = 192;
 static constexpr double kFp32PassReduction = 1e-6;
 // bottom solve: the dense FP64 inverse up to this many unknowns (one
 // bandwidth-bound GEMV per V-cycle), the nested-dissection Cholesky
-// (bottom_nd.hip) above it; SPG_BOTTOM=dense|nd overrides (dense stays
-// limited to kMaxDenseBottomDofs: its inverse is O(n^2))
+// (bottom_nd.hip) above it, unless spg_config.bottom_dense_max sets another
+// limit; SPG_BOTTOM=dense|nd overrides both (dense stays limited to
+// kMaxDenseBottomDofs: its inverse is O(n^2)).  Per density update the
+// dense inverse costs 7, 22, 70 and 192 ms at 1.9 K, 4.6 K, 9 K and 15 K
+// unknowns, the nested dissection 8, 19, 37 and 62 ms, which then costs
+// about 0.08 ms more per V-cycle at 9 K: the dense inverse pays when many
+// solves share a factorization.
 static constexpr int64_t kDenseBottomDofs = 16384;
 static constexpr int64_t kMaxDenseBottomDofs = 60000;
 
@@ -383,8 +388,11 @@ void Solver::build_operators() {
       nv += 3;
     }
   nvar_ = nv;
+  if (trace) std::fprintf(stderr, "[spg] operators bottom unknowns        %lld\n", (long long)nv);
   const char* bsel = std::getenv("SPG_BOTTOM");
-  const bool want_nd = bsel ? std::string(bsel) == "nd" : nv > kDenseBottomDofs;
+  const int64_t dense_max =
+      std::min(cfg_.bottom_dense_max > 0 ? (int64_t)cfg_.bottom_dense_max : kDenseBottomDofs, kMaxDenseBottomDofs);
+  const bool want_nd = bsel ? std::string(bsel) == "nd" : nv > dense_max;
   if (!want_nd && nv > kMaxDenseBottomDofs)
     throw SpgError(SPG_E_INVALID, "bottom level has " + std::to_string(nv) + " unknowns, more than the dense "
                                       "inverse takes (" + std::to_string(kMaxDenseBottomDofs) + "); SPG_BOTTOM=dense "

@@ -83,3 +83,24 @@ def test_nd_bottom_on_a_46k_unknown_bottom(monkeypatch):
     assert _solution_error(P, out, u_o) < 1e-6
     # the dense inverse's FP64 and FP32 tiles hold about 1.5 n^2 * 8 bytes
     assert nd_bytes < 0.25 * 12.0 * nv * nv
+
+
+def test_bottom_dense_max_picks_the_bottom(monkeypatch):
+    """spg_config.bottom_dense_max: a bottom one unknown above the limit gets
+    the nested-dissection Cholesky, one at the limit the dense inverse; both
+    solve to the same FP64 solution"""
+    monkeypatch.delenv("SPG_BOTTOM", raising=False)
+    prob = W.random_blocky(seed=5, n=(24, 16, 16), mg_levels=3)
+    s0 = _lib.Solver(prob)
+    nv = s0.info()["bottom_dofs"]
+    assert 0 < nv <= 16384 and s0.info()["bottom_kind"] == 0  # the default limit
+    s0.close()
+    outs = []
+    for limit, kind in ((nv - 1, 1), (nv, 0)):
+        s = _lib.Solver(prob, bottom_dense_max=limit)
+        assert s.info()["bottom_kind"] == kind, (limit, nv)
+        out, info = s.solve(tolerance=1e-10, max_iterations=500, precision=1)
+        assert info["converged"] == 1
+        outs.append(out)
+        s.close()
+    assert rel(outs[0], outs[1]) < 1e-8

@@ -712,6 +712,22 @@ def test_fem_solve_blocks_off_the_4_grid():
     assert err < 1e-8, err
 
 
+def test_fem_solve_short_of_the_tolerance_still_succeeds(box3):
+    """A solve stopped by max_iterations before the tolerance reports
+    success = 1, as the reference's does (SPGrid_CPU_Solver.cpp:326,368;
+    its caller aborts on success = 0, objective/minimal_compliance.cpp:64-67),
+    with error_code SPG_E_NOT_CONVERGED, the residual it reached, and the
+    displacement of its last iterate."""
+    inp = FI.FEMInputs.from_problem(box3.p, tolerance=1e-12, max_iterations=2)
+    outp = FI.FEMInterface(inp).run()
+    assert outp.success, outp.info
+    assert outp.error_code == _lib.SPG_E_NOT_CONVERGED
+    assert "tolerance not reached" in outp.info
+    assert outp.residuals and 1e-12 < outp.residuals[-1] < 1.0
+    d = outp.displacements[..., :3]
+    assert np.isfinite(d).all() and np.abs(d).max() > 0
+
+
 def test_sensitivity_matches_oracle(box3):
     torch = pytest.importorskip("torch")
     P = box3
