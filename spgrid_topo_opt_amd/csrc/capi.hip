@@ -456,7 +456,12 @@ void fem_solve(void* input, void* output) {
     trace("forces");
     spg_solve_params prm{};
     prm.tolerance = in.krylov.tolerance;
-    prm.max_iterations = in.krylov.max_iterations > 0 ? in.krylov.max_iterations : 500;
+    // the reference's CG runs for at most 500 iterations whatever
+    // krylov.max_iterations says (SPGrid_CPU_Solver.cpp:326; its caller
+    // sends cg_max_iterations, default 50, spgrid_topo_opt.cpp:1000);
+    // extra_ints["pcg_max_iterations"] sets another cap per pass
+    const int cap = in.get_int("pcg_max_iterations", 0);
+    prm.max_iterations = cap > 0 ? cap : 500;
     const int dci = in.get_int("defect_correction_iter", 0);
     prm.max_passes = dci > 0 ? dci : 1;
     prm.precision = in.get_int("fp64_solve", 0) ? 1 : 0;

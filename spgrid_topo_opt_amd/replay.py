@@ -4,8 +4,10 @@
 The optimizer dumps every solve's serialized FEMInputs
 (topo_opt/spgrid_topo_opt.cpp:1206-1222).  This tool reads one dump, applies
 the same command-line overrides as the reference task (`cg_max_iterations=N`,
-`minimum_stiffness=X`; any other field is an error), clears the solver-state
-pointer and runs the drop-in `fem_solve` of this library, or, with
+`minimum_stiffness=X`; any other field is an error; like the reference
+solver, `fem_solve` ignores the krylov.max_iterations that
+`cg_max_iterations` sets), clears the solver-state pointer and runs the
+drop-in `fem_solve` of this library, or, with
 `--convert`, reports the density range and NaN/Inf checks of
 `convert_fem_solve` and writes a human-readable parameter dump.
 

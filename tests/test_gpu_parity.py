@@ -713,12 +713,15 @@ def test_fem_solve_blocks_off_the_4_grid():
 
 
 def test_fem_solve_short_of_the_tolerance_still_succeeds(box3):
-    """A solve stopped by max_iterations before the tolerance reports
+    """A solve stopped by its iteration cap before the tolerance reports
     success = 1, as the reference's does (SPGrid_CPU_Solver.cpp:326,368;
     its caller aborts on success = 0, objective/minimal_compliance.cpp:64-67),
     with error_code SPG_E_NOT_CONVERGED, the residual it reached, and the
-    displacement of its last iterate."""
-    inp = FI.FEMInputs.from_problem(box3.p, tolerance=1e-12, max_iterations=2)
+    displacement of its last iterate.  krylov.max_iterations is ignored, as
+    by the reference (500 iterations); the cap here is the
+    pcg_max_iterations extra."""
+    inp = FI.FEMInputs.from_problem(box3.p, tolerance=1e-12, max_iterations=2,
+                                    extra_ints={"pcg_max_iterations": 2})
     outp = FI.FEMInterface(inp).run()
     assert outp.success, outp.info
     assert outp.error_code == _lib.SPG_E_NOT_CONVERGED
@@ -726,6 +729,13 @@ def test_fem_solve_short_of_the_tolerance_still_succeeds(box3):
     assert outp.residuals and 1e-12 < outp.residuals[-1] < 1.0
     d = outp.displacements[..., :3]
     assert np.isfinite(d).all() and np.abs(d).max() > 0
+    # krylov.max_iterations alone (the reference caller's default 50 here)
+    # does not cap the solve
+    outp = FI.FEMInterface(FI.FEMInputs.from_problem(box3.p, tolerance=1e-10, max_iterations=2,
+                                                     extra_ints={"defect_correction_iter": 20},
+                                                     extra_doubles={"inner_tolerance": 1e-3})).run()
+    assert outp.success and outp.error_code == 0, outp.info
+    assert outp.residuals[-1] <= 1e-10
 
 
 def test_sensitivity_matches_oracle(box3):
