@@ -41,7 +41,10 @@ class FEM:
         tolerance is relative to that first residual, as the reference's
         kept state does (SPGrid_CPU_Solver.cpp:247-285, 318-326)."""
         self.prob = prob
-        self.solver = _lib.Solver(prob, device=device)
+        # a density update refactors the bottom for about one solve, so the
+        # nested-dissection Cholesky takes bottoms above 4096 unknowns, as in
+        # fem_solve (it factors faster than the dense inverse there)
+        self.solver = _lib.Solver(prob, device=device, bottom_dense_max=4096)
         self.nin = prob.block_base.shape[0] * 64
         self.kw = dict(tolerance=tolerance, precision=precision, inner_tolerance=inner_tolerance if precision == 0
                        else 0.0, max_passes=max_passes if precision == 0 else 1, max_iterations=max_iterations)
