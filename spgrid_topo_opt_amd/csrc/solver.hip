@@ -587,10 +587,36 @@ void Solver::update_density(const double* density) {
         }
       },
       1 << 15);
-  HIPCHK(hipMemcpy(E64_.p, lay_.E.data(), lay_.E.size() * 8, hipMemcpyHostToDevice));
+  // both arrays through the kept pinned buffer, filled on host threads: a
+  // pageable copy runs at a few GB/s on one thread
+  const size_t ne = lay_.E.size() * 8, nr = lay_.rho.size() * 8;
+  HIPCHK(hipStreamSynchronize(st_));  // the buffer may still feed an earlier copy
+  if (stage_bytes_ < ne + nr) {
+    if (stage_) (void)hipHostFree(stage_);
+    stage_ = nullptr;
+    stage_bytes_ = 0;
+    HIPCHK(hipHostMalloc(&stage_, ne + nr));
+    stage_bytes_ = ne + nr;
+  }
+  uint8_t* hs = static_cast<uint8_t*>(stage_);
+  const uint8_t* srcE = reinterpret_cast<const uint8_t*>(lay_.E.data());
+  const uint8_t* srcR = reinterpret_cast<const uint8_t*>(lay_.rho.data());
+  parallel_ranges(
+      (int64_t)(ne + nr),
+      [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b;) {
+          const int64_t end = i < (int64_t)ne ? std::min<int64_t>(b, ne) : b;
+          std::memcpy(hs + i, i < (int64_t)ne ? srcE + i : srcR + (i - ne), (size_t)(end - i));
+          i = end;
+        }
+      },
+      1 << 22);
+  HIPCHK(hipMemcpyAsync(E64_.p, hs, ne, hipMemcpyHostToDevice, st_));
+  HIPCHK(hipMemcpyAsync(rho_.p, hs + ne, nr, hipMemcpyHostToDevice, st_));
   launch_cast_d2f(st_, E64_.as<double>(), E32_.as<float>(), (int64_t)lay_.E.size());
-  HIPCHK(hipMemcpy(rho_.p, lay_.rho.data(), lay_.rho.size() * 8, hipMemcpyHostToDevice));
   build_operators();
+  // the pinned buffer must not change before the copies have read it
+  HIPCHK(hipStreamSynchronize(st_));
 }
 
 void Solver::update_density_device(const double* density_dev) {
